@@ -1,0 +1,415 @@
+// bfs_device.hpp -- gfx950 device building blocks of the BFS kernels.
+//
+// Wave64 primitives (DPP prefix sum, ballot/mbcnt compaction), the per-wave
+// LDS-staged frontier queue, the visited-bitmask claims, the three-tier
+// neighbour gather and the hierarchical grid barrier.  These replace the
+// reference's CTA raking scans, warp scans, tile I/O, culls and software
+// global barrier (b40c/util/scan/*.cuh, b40c/util/io/*.cuh,
+// b40c/util/global_barrier.cuh, b40c/graph/bfs/*/cta.cuh); the designs are
+// written for 64-lane waves and CDNA memory scopes, not translated.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace b40c_hip {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;                    // threads per workgroup
+constexpr int kWaves = kBlock / kWave;         // waves per workgroup
+constexpr int kWaveQueueCap = 512;             // LDS staging entries per wave
+constexpr int kHistory = 1024;                 // LDS history-hash entries per workgroup
+constexpr int kBarrierFanIn = 16;              // first-level barrier counters
+constexpr int kHubChunk = 2048;                // edges per grid-tier chunk
+constexpr int kLevelLog = 64;                  // steps with recorded length/clock
+
+// ---------------------------------------------------------------------------
+// Device control block: frontier lengths, tile counters, barrier state, exit
+// status.  Counters rotate by step so a step never zeroes a counter that is
+// still being read: step s reads qlen[s % 3], appends to qlen[(s+1) % 3] and
+// zeroes qlen[(s+2) % 3] and steal[(s+1) & 1].
+// ---------------------------------------------------------------------------
+struct alignas(256) Control {
+    unsigned qlen[4];
+    unsigned steal[2];
+    unsigned overflow;
+    int exit_reason;          // 0 done, 1 saturated, 2 overflow
+    int exit_step;            // loop position when a persistent launch ends
+    int exit_iteration;
+    int exit_phase;
+    int fast_step;            // single-workgroup fast path hand-off
+    int fast_iteration;
+    int fast_phase;
+    int max_depth;            // deepest label written
+    int pad0[1];
+    unsigned long long total_queued;
+    unsigned long long busy_cycles;   // instrumented: sum over workgroups of step time
+    unsigned long long life_cycles;   // instrumented: sum over workgroups of kernel time
+    unsigned long long hub_alloc[2];  // grid tier: (entries << 32) | chunks, by step & 1
+    unsigned hub_steal[2];            // grid tier: chunk counter, by step & 1
+    unsigned long long pad1[2];
+    unsigned bar_root;        // own 128-byte line
+    unsigned pad3[31];
+    unsigned bar_leaf[kBarrierFanIn * 32];     // one 128-byte line per leaf counter
+    unsigned bar_release[kBarrierFanIn * 32];  // one 128-byte line per leaf release word
+    unsigned level_len[kLevelLog];             // input length of each step
+    unsigned long long level_clock[kLevelLog]; // wall clock at the start of each step
+};
+
+// Grid-tier work item: a row too long for one workgroup, split into
+// kHubChunk-edge chunks [first_chunk, first_chunk + ceil(len / kHubChunk)).
+struct HubEntry {
+    int beg, end, par;
+    unsigned first_chunk;
+};
+
+enum ExitReason { kExitDone = 0, kExitSaturated = 1, kExitOverflow = 2 };
+
+// ---------------------------------------------------------------------------
+// Wave64 primitives
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id()
+{
+    return (int) __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// Number of set bits of `m` below the calling lane.
+__device__ __forceinline__ int lanes_below(uint64_t m)
+{
+    return (int) __builtin_amdgcn_mbcnt_hi((unsigned) (m >> 32),
+                                           __builtin_amdgcn_mbcnt_lo((unsigned) m, 0u));
+}
+
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ int readlane(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+
+// Inclusive prefix sum across the 64 lanes with DPP row shifts inside each
+// 16-lane row followed by the row_bcast:15 / row_bcast:31 carries between
+// rows (GFX9/CDNA DPP controls 0x111..0x118, 0x142, 0x143).  Lanes whose
+// DPP source is out of the row, or whose row is masked off, add 0.
+__device__ __forceinline__ int wave_inclusive_sum(int x)
+{
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);  // row_bcast:15 -> rows 1, 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);  // row_bcast:31 -> rows 2, 3
+    return x;
+}
+
+// ---------------------------------------------------------------------------
+// Scoped memory helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned ld_agent(const unsigned *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_agent(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(unsigned *p, unsigned v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(int *p, int v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned add_agent(unsigned *p, unsigned v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// Visited claims.
+//
+// Exact: the bitmask word is first read with a plain (cacheable, possibly
+// stale) load.  Bits only ever go 0 -> 1 during a search, so a set bit is
+// always true and skips the atomic; a clear bit may be stale and is settled
+// by the agent-scope atomic OR, whose returned old value makes exactly one
+// claimant the owner of the vertex.
+//
+// Best effort (the reference discipline, contract_expand_atomic/cta.cuh:
+// 213-304): racy read-modify-write of the mask byte as a pre-filter, then the
+// label check; racing discoverers may both succeed, which only duplicates
+// work because they write the same depth.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool claim_exact(unsigned *visited, int u)
+{
+    unsigned *w = visited + (u >> 5);
+    unsigned bit = 1u << (u & 31);
+    if (*w & bit) return false;
+    unsigned old = __hip_atomic_fetch_or(w, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return (old & bit) == 0;
+}
+
+__device__ __forceinline__ bool claim_best_effort(unsigned *visited, const int *depth, int u)
+{
+    unsigned *w = visited + (u >> 5);
+    unsigned bit = 1u << (u & 31);
+    unsigned cur = *w;
+    if (cur & bit) return false;
+    *w = cur | bit;
+    return depth[u] == -1;
+}
+
+template <bool EXACT>
+__device__ __forceinline__ bool claim(unsigned *visited, const int *depth, int u)
+{
+    if (EXACT) return claim_exact(visited, u);
+    return claim_best_effort(visited, depth, u);
+}
+
+// ---------------------------------------------------------------------------
+// LDS layout shared by every BFS kernel.
+// ---------------------------------------------------------------------------
+template <bool PRED>
+struct PredStore {
+    int buf[kWaves][kWaveQueueCap];
+};
+template <>
+struct PredStore<false> {
+    int buf[1][1];
+};
+
+template <bool PRED>
+struct Lds {
+    int wq[kWaves][kWaveQueueCap];       // per-wave frontier staging
+    PredStore<PRED> wqp;                 // per-wave predecessor staging
+    int sc_excl[kWaves][kWave];          // scan tier: exclusive rank per lane
+    int sc_beg[kWaves][kWave];           // scan tier: row begin per lane
+    int sc_par[kWaves][kWave];           // scan tier: row owner per lane
+    int hist[kHistory];                  // history hash (contract / filter)
+    int cta_owner, cta_beg, cta_end, cta_par;
+    unsigned tile;
+    unsigned nb;                         // broadcast frontier length
+    int step, iteration;
+};
+
+// ---------------------------------------------------------------------------
+// Per-wave frontier queue.  Lanes append through ballot + mbcnt into the
+// wave's LDS staging area; a full area (or the end of a step) is flushed with
+// one agent-scope atomic reservation and coalesced stores.  One global atomic
+// per kWaveQueueCap appended elements (or per wave per step) replaces the
+// reference's per-tile raking scan + atomicAdd (util/scan/cooperative_scan.cuh:
+// 172-195, 309-343).
+// ---------------------------------------------------------------------------
+template <bool PRED>
+struct WaveQueue {
+    int *buf;
+    int *pbuf;
+    int count;          // wave-uniform
+    int *out;
+    int *pout;
+    unsigned *out_len;
+    unsigned cap;
+    unsigned *overflow;
+
+    __device__ __forceinline__ void flush()
+    {
+        if (count == 0) return;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        int lane = lane_id();
+        unsigned base = 0;
+        if (lane == 0) base = add_agent(out_len, (unsigned) count);
+        base = (unsigned) uniform((int) base);
+        unsigned lim = 0;
+        if (base < cap) lim = min((unsigned) count, cap - base);
+        if (lim < (unsigned) count && lane == 0) st_agent(overflow, 1u);
+        for (unsigned i = lane; i < lim; i += kWave) {
+            out[base + i] = buf[i];
+            if (PRED) pout[base + i] = pbuf[i];
+        }
+        count = 0;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+
+    // Wave-converged: every lane of the wave calls push together.
+    __device__ __forceinline__ void push(bool flag, int u, int p)
+    {
+        uint64_t m = ballot(flag);
+        if (m == 0) return;
+        int c = __popcll(m);
+        if (count + c > kWaveQueueCap) flush();
+        if (flag) {
+            int pos = count + lanes_below(m);
+            buf[pos] = u;
+            if (PRED) pbuf[pos] = p;
+        }
+        count += c;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Three-tier neighbour gather.  Each thread owns one adjacency row
+// [beg, end) with owner `par` (end == beg for none).  fn(valid, u, par) is
+// called wave-converged for every neighbour u exactly once:
+//   CTA tier  (len >= cta_t):  the whole workgroup strides through the row;
+//   wave tier (len >= wave_t): one wave strides through the row;
+//   scan tier (len <  wave_t): a wave-wide DPP prefix sum ranks the short
+//       rows, and each lane locates its edge by a 6-step search over the 64
+//       LDS-staged exclusive ranks, so the wave's lanes stay fully occupied
+//       whatever the mix of short lengths.
+// Replaces ExpandByCta / ExpandByWarp / ExpandByScan
+// (b40c/graph/bfs/two_phase/expand_atomic/cta.cuh:215-423).
+// ---------------------------------------------------------------------------
+struct HubList {
+    HubEntry *entries;
+    unsigned cap;
+    unsigned long long *alloc;
+    int grid_t;        // rows at least this long go to the grid tier
+};
+
+template <bool PRED, typename Fn>
+__device__ __forceinline__ void gather_rows(int beg, int end, int par, const int *__restrict__ col,
+                                            int wave_t, int cta_t, const HubList &hub, Lds<PRED> &lds, Fn &&fn)
+{
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    int len = end - beg;
+
+    // Grid tier: publish the row as chunks for every workgroup of the grid
+    // (processed after this tile phase).  A full list falls back to the CTA
+    // tier below.
+    if (len >= hub.grid_t) {
+        unsigned long long nch = (unsigned) ((len + kHubChunk - 1) / kHubChunk);
+        unsigned long long old = __hip_atomic_fetch_add(hub.alloc, (1ull << 32) | nch, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+        unsigned slot = (unsigned) (old >> 32);
+        if (slot < hub.cap) {
+            hub.entries[slot] = HubEntry{beg, end, par, (unsigned) old};
+            len = 0;
+        }
+    }
+
+    // CTA tier
+    while (true) {
+        __syncthreads();
+        if (len >= cta_t) lds.cta_owner = tid;
+        __syncthreads();
+        int owner = lds.cta_owner;
+        if (owner < 0) break;
+        if (tid == owner) {
+            lds.cta_beg = beg;
+            lds.cta_end = end;
+            lds.cta_par = par;
+            len = 0;
+        }
+        __syncthreads();
+        int b = lds.cta_beg, e = lds.cta_end, pv = lds.cta_par;
+        __syncthreads();
+        if (tid == 0) lds.cta_owner = -1;
+        for (int k = b; k < e; k += kBlock) {
+            int idx = k + tid;
+            bool ok = idx < e;
+            int u = ok ? __builtin_nontemporal_load(col + idx) : -1;
+            fn(ok, u, pv);
+        }
+    }
+
+    // Wave tier
+    uint64_t wm = ballot(len >= wave_t);
+    while (wm) {
+        int l = uniform(__builtin_ctzll(wm));
+        wm &= wm - 1;
+        int b = readlane(beg, l), e = readlane(end, l), pv = readlane(par, l);
+        for (int k = b; k < e; k += kWave) {
+            int idx = k + lane;
+            bool ok = idx < e;
+            int u = ok ? __builtin_nontemporal_load(col + idx) : -1;
+            fn(ok, u, pv);
+        }
+    }
+    if (len >= wave_t) len = 0;
+
+    // Scan tier
+    int incl = wave_inclusive_sum(len);
+    int total = readlane(incl, kWave - 1);
+    if (total == 0) return;
+    int *excl = lds.sc_excl[wave];
+    int *rb = lds.sc_beg[wave];
+    int *rp = lds.sc_par[wave];
+    excl[lane] = incl - len;
+    rb[lane] = beg;
+    rp[lane] = par;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int r = 0; r < total; r += kWave) {
+        int idx = r + lane;
+        bool ok = idx < total;
+        int u = -1, pv = -1;
+        if (ok) {
+            int l = 0;
+#pragma unroll
+            for (int s = 32; s > 0; s >>= 1)
+                if (excl[l + s] <= idx) l += s;
+            u = __builtin_nontemporal_load(col + rb[l] + (idx - excl[l]));
+            pv = rp[l];
+        }
+        fn(ok, u, pv);
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// ---------------------------------------------------------------------------
+// Grid barrier for cooperatively launched (co-resident) grids.
+// Two-level arrival: workgroup b arrives on leaf counter b % kBarrierFanIn;
+// the last arrival at a leaf arrives at the root; the last root arrival
+// resets the counters and publishes the new generation into every leaf's
+// release word, so waiting workgroups poll kBarrierFanIn different lines
+// instead of one.
+//
+// Memory ordering costs one agent-scope release fence (L2 write-back) and one
+// acquire fence (L1/L2 invalidate) per workgroup, issued by thread 0 around
+// the workgroup barriers.  Polling uses relaxed loads: an acquire load inside
+// the spin would invalidate the caches on every poll and slow every other
+// workgroup still working.  The last arrivers chain acquire/release fences so
+// every workgroup's pre-barrier writes happen-before every post-barrier read.
+// Replaces util::GlobalBarrier (b40c/util/global_barrier.cuh:80-126).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void grid_barrier(Control *ctl)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned nblocks = gridDim.x;
+        const unsigned leaf = blockIdx.x % kBarrierFanIn;
+        const unsigned leaves = nblocks < (unsigned) kBarrierFanIn ? nblocks : (unsigned) kBarrierFanIn;
+        // Workgroups on this leaf: ceil((nblocks - leaf) / fan_in).
+        const unsigned leaf_size = (nblocks - leaf + kBarrierFanIn - 1) / kBarrierFanIn;
+        unsigned *release = &ctl->bar_release[leaf * 32];
+        unsigned gen = __hip_atomic_load(release, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        unsigned *lc = &ctl->bar_leaf[leaf * 32];
+        unsigned arrived = __hip_atomic_fetch_add(lc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bool last = false;
+        if (arrived == leaf_size - 1) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+            __hip_atomic_store(lc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            unsigned r = __hip_atomic_fetch_add(&ctl->bar_root, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (r == leaves - 1) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+                __hip_atomic_store(&ctl->bar_root, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (unsigned l = 0; l < leaves; l++)
+                    __hip_atomic_store(&ctl->bar_release[l * 32], gen + 1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                last = true;
+            }
+        }
+        if (!last) {
+            while (__hip_atomic_load(release, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
+                __builtin_amdgcn_s_sleep(2);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
+}  // namespace b40c_hip

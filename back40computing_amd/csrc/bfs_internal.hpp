@@ -1,0 +1,70 @@
+// bfs_internal.hpp -- types shared by the kernel translation unit and the
+// C-ABI translation unit.  Not installed; the public interface is
+// include/b40c_bfs.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace b40c_hip {
+
+struct Control;
+struct HubEntry;
+
+// Phases of one step of the level loop.
+enum Phase : int {
+    kPhaseExpandContract = 0,  // vertex frontier -> vertex frontier (gather + claim)
+    kPhaseContractExpand = 1,  // edge frontier -> edge frontier (claim + gather)
+    kPhaseContract = 2,        // edge frontier -> vertex frontier (claim)
+    kPhaseExpand = 3,          // vertex frontier -> edge frontier (gather)
+    kPhaseFilter = 4           // edge frontier -> edge frontier (visited-bitmask cull)
+};
+
+struct GraphArgs {
+    const int *row;   // [V + 1]
+    const int *col;   // [E]
+    int nodes;
+};
+
+struct SearchArgs {
+    int *depth;            // [V]
+    int *pred;             // [V] or null
+    unsigned *visited;     // [ceil(V / 32)]
+    int *q[2];             // ping-pong frontier queues
+    int *pq[2];            // predecessor queues (null unless marking predecessors)
+    unsigned cap;          // capacity of each queue, in elements
+    Control *ctl;
+    HubEntry *hubs[2];     // grid-tier lists, by step & 1
+    unsigned hub_cap;      // entries per list
+    int wave_t, cta_t, grid_t, fast_t, sat_t;
+    int instrument;
+};
+
+// Loop position: step index (selects queues and counters), BFS iteration
+// (depth written by the next claim), and phase (two-phase cycle).
+struct Cursor {
+    int step;
+    int iteration;
+    int phase;
+};
+
+// Strategy of a persistent launch.
+enum LoopKind : int { kLoopEC = 0, kLoopCE = 1, kLoopTP = 2 };
+
+// Host launchers (bfs_kernels.hip).
+hipError_t launch_init_search(const SearchArgs &s, int src, int nodes, int label_source, hipStream_t st);
+hipError_t launch_reset_labels(int *depth, int *pred, unsigned *visited, int nodes, hipStream_t st);
+// One step as a kernel launch; gathering phases are followed by the
+// grid-tier launch for rows published to the hub list.
+hipError_t launch_step(int phase, bool pred, bool exact, const GraphArgs &g, const SearchArgs &s, Cursor c,
+                       int grid, hipStream_t st, int *launches);
+hipError_t launch_persistent(int loop, bool pred, bool exact, const GraphArgs &g, const SearchArgs &s, Cursor c,
+                             int grid, hipStream_t st);
+// Largest co-resident grid of the persistent kernel for `loop` (0 on error).
+int persistent_max_blocks_per_cu(int loop, bool pred, bool exact);
+int step_max_blocks_per_cu(bool pred);
+hipError_t launch_barrier_bench(Control *ctl, int iterations, int grid, hipStream_t st);
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
+hipError_t launch_wave_selftest(const int *in, int *out_scan, int *out_rank, int n, hipStream_t st);
+
+}  // namespace b40c_hip

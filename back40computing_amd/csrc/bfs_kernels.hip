@@ -1,0 +1,653 @@
+// bfs_kernels.hip -- gfx950 breadth-first-search kernels.
+//
+// One step of the level loop maps a frontier queue to the next one.  The
+// step bodies are:
+//   expand-contract  vertex frontier -> gather neighbours -> claim -> vertex
+//                    frontier (b40c/graph/bfs/expand_contract_atomic/cta.cuh:665-788)
+//   contract-expand  edge frontier -> claim -> gather neighbours -> edge
+//                    frontier (b40c/graph/bfs/contract_expand_atomic/cta.cuh:573-709)
+//   contract         edge frontier -> claim -> vertex frontier
+//                    (b40c/graph/bfs/two_phase/contract_atomic/cta.cuh:494-572)
+//   expand           vertex frontier -> gather -> edge frontier
+//                    (b40c/graph/bfs/two_phase/expand_atomic/cta.cuh:606-719)
+//   filter           edge frontier -> visited-bitmask cull -> edge frontier
+//                    (b40c/graph/bfs/two_phase/filter_atomic/cta.cuh:308-388)
+// Gathering steps have two sub-phases: the tile phase (frontier tiles taken
+// dynamically by workgroups, rows gathered by the CTA / wave / scan tiers)
+// and, when some rows were long enough for the grid tier, the hub phase in
+// which every workgroup takes fixed-size chunks of those rows.
+// Each step runs either inside a persistent, cooperatively launched loop with
+// in-kernel grid barriers (the reference's KernelGlobalBarrier kernels) or
+// as kernel launches per step.
+#include "bfs_device.hpp"
+#include "bfs_internal.hpp"
+
+namespace b40c_hip {
+
+template <bool PRED>
+__device__ __forceinline__ void init_lds(Lds<PRED> &lds)
+{
+    for (int i = threadIdx.x; i < kHistory; i += kBlock) lds.hist[i] = -1;
+    if (threadIdx.x == 0) lds.cta_owner = -1;
+    __syncthreads();
+}
+
+// Dynamic work assignment: one agent-scope atomic per tile (or chunk).
+template <bool PRED>
+__device__ __forceinline__ unsigned next_item(unsigned *counter, Lds<PRED> &lds)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) lds.tile = add_agent(counter, 1u);
+    __syncthreads();
+    return lds.tile;
+}
+
+// LDS history hash: true if u may be new to this workgroup.  A hit means
+// another lane of this workgroup already took u through the claim, so this
+// copy can be dropped without touching global memory.  Racing lanes may both
+// miss; the claim that follows settles them.  Cf. HistoryCull / WarpCull
+// (b40c/graph/bfs/two_phase/contract_atomic/cta.cuh:276-327).
+template <bool PRED>
+__device__ __forceinline__ bool history_pass(Lds<PRED> &lds, int u)
+{
+    unsigned h = ((unsigned) u * 0x9E3779B1u) >> (32 - 10);
+    static_assert(kHistory == 1024, "history hash uses 10 bits");
+    if (lds.hist[h] == u) return false;
+    lds.hist[h] = u;
+    return true;
+}
+
+template <bool PRED>
+__device__ __forceinline__ WaveQueue<PRED> make_queue(const SearchArgs &s, int step, Lds<PRED> &lds)
+{
+    const int wave = threadIdx.x / kWave;
+    WaveQueue<PRED> wq;
+    wq.buf = lds.wq[wave];
+    if constexpr (PRED) wq.pbuf = lds.wqp.buf[wave];
+    else wq.pbuf = nullptr;
+    wq.count = 0;
+    wq.out = s.q[(step + 1) & 1];
+    wq.pout = PRED ? s.pq[(step + 1) & 1] : nullptr;
+    wq.out_len = &s.ctl->qlen[(step + 1) % 3];
+    wq.cap = s.cap;
+    wq.overflow = &s.ctl->overflow;
+    return wq;
+}
+
+__device__ __forceinline__ HubList hub_list(const SearchArgs &s, int step, bool enabled)
+{
+    HubList h;
+    h.entries = s.hubs[step & 1];
+    h.cap = s.hub_cap;
+    h.alloc = &s.ctl->hub_alloc[step & 1];
+    h.grid_t = enabled ? s.grid_t : 0x7fffffff;
+    return h;
+}
+
+// Records the deepest label written by this wave in this step.
+__device__ __forceinline__ void note_depth(Control *ctl, bool labeled, int d)
+{
+    if (ballot(labeled) != 0 && lane_id() == 0) atomicMax(&ctl->max_depth, d);
+}
+
+// ---------------------------------------------------------------------------
+// Generic phases
+// ---------------------------------------------------------------------------
+
+// Tile phase of a gathering step.  row_of(i, beg, end, par) turns input
+// element i (< n) into the adjacency row to gather (end == beg for none);
+// edge(ok, u, par) consumes each neighbour, wave-converged.
+template <bool PRED, typename RowOf, typename Edge>
+__device__ __forceinline__ void tile_phase(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n,
+                                           const HubList &hub, Lds<PRED> &lds, RowOf &&row_of, Edge &&edge)
+{
+    while (true) {
+        unsigned base = next_item(&s.ctl->steal[c.step & 1], lds) * kBlock;
+        if (base >= n) break;
+        unsigned i = base + threadIdx.x;
+        int beg = 0, end = 0, par = -1;
+        if (i < n) row_of(i, beg, end, par);
+        gather_rows<PRED>(beg, end, par, g.col, s.wave_t, s.cta_t, hub, lds, edge);
+    }
+}
+
+// Hub phase: every workgroup takes kHubChunk-edge chunks of the rows that
+// the tile phase published, until all chunks are done.  Must follow a grid
+// barrier (or kernel boundary) after the tile phase.
+template <bool PRED, typename Edge>
+__device__ __forceinline__ void hub_phase(const GraphArgs &g, const SearchArgs &s, Cursor c, Lds<PRED> &lds,
+                                          Edge &&edge)
+{
+    unsigned long long a = __hip_atomic_load(&s.ctl->hub_alloc[c.step & 1], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    unsigned entries = (unsigned) (a >> 32);
+    if (entries > s.hub_cap) entries = s.hub_cap;
+    if (entries == 0) return;
+    const HubEntry *list = s.hubs[c.step & 1];
+    HubEntry lastent = list[entries - 1];
+    unsigned total = lastent.first_chunk + (unsigned) ((lastent.end - lastent.beg + kHubChunk - 1) / kHubChunk);
+    while (true) {
+        unsigned ch = next_item(&s.ctl->hub_steal[c.step & 1], lds);
+        if (ch >= total) break;
+        if (threadIdx.x == 0) {
+            // last entry with first_chunk <= ch (entries are in chunk order)
+            unsigned lo = 0, hi = entries - 1;
+            while (lo < hi) {
+                unsigned mid = (lo + hi + 1) >> 1;
+                if (list[mid].first_chunk <= ch) lo = mid;
+                else hi = mid - 1;
+            }
+            HubEntry e = list[lo];
+            int b = e.beg + (int) (ch - e.first_chunk) * kHubChunk;
+            lds.cta_beg = b;
+            lds.cta_end = min(e.end, b + kHubChunk);
+            lds.cta_par = e.par;
+        }
+        __syncthreads();
+        int b = lds.cta_beg, e = lds.cta_end, pv = lds.cta_par;
+        for (int k = b; k < e; k += kBlock) {
+            int idx = k + threadIdx.x;
+            bool ok = idx < e;
+            int u = ok ? __builtin_nontemporal_load(g.col + idx) : -1;
+            edge(ok, u, pv);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Step bodies.  All threads of the workgroup execute them together.
+// sub == 0 runs the tile phase, sub == 1 the hub phase.
+// ---------------------------------------------------------------------------
+template <bool PRED, bool EXACT>
+__device__ void body_expand_contract(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds,
+                                     int sub, bool hubs)
+{
+    WaveQueue<PRED> wq = make_queue<PRED>(s, c.step, lds);
+    const int *in = s.q[c.step & 1];
+    const int next_depth = c.iteration + 1;
+    bool labeled = false;
+    auto edge = [&](bool ok, int u, int par) {
+        bool fresh = ok && claim<EXACT>(s.visited, s.depth, u);
+        if (fresh) {
+            s.depth[u] = next_depth;
+            if (PRED) s.pred[u] = par;
+            labeled = true;
+        }
+        wq.push(fresh, u, par);
+    };
+    if (sub == 0) {
+        tile_phase<PRED>(g, s, c, n, hub_list(s, c.step, hubs), lds, [&](unsigned i, int &beg, int &end, int &par) {
+            int v = in[i];
+            beg = g.row[v];
+            end = g.row[v + 1];
+            par = v;
+        }, edge);
+    } else {
+        hub_phase<PRED>(g, s, c, lds, edge);
+    }
+    wq.flush();
+    note_depth(s.ctl, labeled, next_depth);
+}
+
+template <bool PRED, bool EXACT>
+__device__ void body_contract_expand(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds,
+                                     int sub, bool hubs)
+{
+    WaveQueue<PRED> wq = make_queue<PRED>(s, c.step, lds);
+    const int *in = s.q[c.step & 1];
+    const int *pin = s.pq[c.step & 1];
+    bool labeled = false;
+    auto edge = [&](bool ok, int w, int par) { wq.push(ok, w, par); };
+    if (sub == 0) {
+        tile_phase<PRED>(g, s, c, n, hub_list(s, c.step, hubs), lds, [&](unsigned i, int &beg, int &end, int &par) {
+            int u = in[i];
+            int p = PRED ? pin[i] : -1;
+            if (history_pass<PRED>(lds, u) && claim<EXACT>(s.visited, s.depth, u)) {
+                s.depth[u] = c.iteration;
+                if (PRED) s.pred[u] = p;
+                beg = g.row[u];
+                end = g.row[u + 1];
+                par = u;
+                labeled = true;
+            }
+        }, edge);
+    } else {
+        hub_phase<PRED>(g, s, c, lds, edge);
+    }
+    wq.flush();
+    note_depth(s.ctl, labeled, c.iteration);
+}
+
+template <bool PRED, bool EXACT>
+__device__ void body_expand(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds, int sub,
+                            bool hubs)
+{
+    WaveQueue<PRED> wq = make_queue<PRED>(s, c.step, lds);
+    const int *in = s.q[c.step & 1];
+    auto edge = [&](bool ok, int w, int par) { wq.push(ok, w, par); };
+    if (sub == 0) {
+        tile_phase<PRED>(g, s, c, n, hub_list(s, c.step, hubs), lds, [&](unsigned i, int &beg, int &end, int &par) {
+            int v = in[i];
+            beg = g.row[v];
+            end = g.row[v + 1];
+            par = v;
+        }, edge);
+    } else {
+        hub_phase<PRED>(g, s, c, lds, edge);
+    }
+    wq.flush();
+}
+
+template <bool PRED, bool EXACT>
+__device__ void body_contract(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds)
+{
+    (void) g;
+    WaveQueue<PRED> wq = make_queue<PRED>(s, c.step, lds);
+    const int *in = s.q[c.step & 1];
+    const int *pin = s.pq[c.step & 1];
+    bool labeled = false;
+    while (true) {
+        unsigned base = next_item(&s.ctl->steal[c.step & 1], lds) * kBlock;
+        if (base >= n) break;
+        unsigned i = base + threadIdx.x;
+        int u = -1, p = -1;
+        if (i < n) {
+            u = in[i];
+            if (PRED) p = pin[i];
+        }
+        bool keep = u >= 0 && history_pass<PRED>(lds, u) && claim<EXACT>(s.visited, s.depth, u);
+        if (keep) {
+            s.depth[u] = c.iteration;
+            if (PRED) s.pred[u] = p;
+            labeled = true;
+        }
+        wq.push(keep, u, -1);
+    }
+    wq.flush();
+    note_depth(s.ctl, labeled, c.iteration);
+}
+
+template <bool PRED, bool EXACT>
+__device__ void body_filter(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds)
+{
+    (void) g;
+    WaveQueue<PRED> wq = make_queue<PRED>(s, c.step, lds);
+    const int *in = s.q[c.step & 1];
+    const int *pin = s.pq[c.step & 1];
+    while (true) {
+        unsigned base = next_item(&s.ctl->steal[c.step & 1], lds) * kBlock;
+        if (base >= n) break;
+        unsigned i = base + threadIdx.x;
+        int u = -1, p = -1;
+        if (i < n) {
+            u = in[i];
+            if (PRED) p = pin[i];
+        }
+        // Bitmask only: the history hash is reserved for claims, because a
+        // hit there must mean the vertex has already been claimed or found
+        // visited by this workgroup.
+        bool keep = false;
+        if (u >= 0) {
+            unsigned w = s.visited[u >> 5];
+            keep = ((w >> (u & 31)) & 1u) == 0;
+        }
+        wq.push(keep, u, p);
+    }
+    wq.flush();
+}
+
+__device__ __forceinline__ bool gathers(int phase)
+{
+    return phase == kPhaseExpandContract || phase == kPhaseContractExpand || phase == kPhaseExpand;
+}
+
+template <bool PRED, bool EXACT>
+__device__ __forceinline__ void run_body(int phase, const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n,
+                                         Lds<PRED> &lds, int sub, bool hubs)
+{
+    switch (phase) {
+    case kPhaseExpandContract: body_expand_contract<PRED, EXACT>(g, s, c, n, lds, sub, hubs); break;
+    case kPhaseContractExpand: body_contract_expand<PRED, EXACT>(g, s, c, n, lds, sub, hubs); break;
+    case kPhaseContract: body_contract<PRED, EXACT>(g, s, c, n, lds); break;
+    case kPhaseExpand: body_expand<PRED, EXACT>(g, s, c, n, lds, sub, hubs); break;
+    default: body_filter<PRED, EXACT>(g, s, c, n, lds); break;
+    }
+}
+
+__device__ __forceinline__ Cursor advance(int loop, Cursor c)
+{
+    c.step++;
+    if (loop != kLoopTP) {
+        c.iteration++;
+    } else {
+        if (c.phase == kPhaseContract) c.phase = kPhaseExpand;
+        else if (c.phase == kPhaseExpand) c.phase = kPhaseFilter;
+        else { c.phase = kPhaseContract; c.iteration++; }
+    }
+    return c;
+}
+
+// Counter maintenance for step c (one thread): zero the counters the next
+// step appends to / steals from, and account the input length.
+__device__ __forceinline__ void prepare_step(Control *ctl, Cursor c, unsigned n)
+{
+    st_agent(&ctl->qlen[(c.step + 2) % 3], 0u);
+    st_agent(&ctl->steal[(c.step + 1) & 1], 0u);
+    st_agent(&ctl->hub_steal[(c.step + 1) & 1], 0u);
+    __hip_atomic_store(&ctl->hub_alloc[(c.step + 1) & 1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ctl->total_queued += n;
+    if (c.step < kLevelLog) {
+        ctl->level_len[c.step] = n;
+        ctl->level_clock[c.step] = wall_clock64();
+    }
+}
+
+__device__ __forceinline__ unsigned frontier_length(const SearchArgs &s, int step)
+{
+    unsigned n = ld_agent(&s.ctl->qlen[step % 3]);
+    return n < s.cap ? n : s.cap;
+}
+
+__device__ __forceinline__ bool hubs_pending(const SearchArgs &s, int step)
+{
+    return __hip_atomic_load(&s.ctl->hub_alloc[step & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull;
+}
+
+// ---------------------------------------------------------------------------
+// Persistent level loop (cooperative launch).
+// ---------------------------------------------------------------------------
+template <bool PRED, bool EXACT>
+__global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchArgs s, Cursor c, int loop)
+{
+    __shared__ Lds<PRED> lds;
+    init_lds(lds);
+    unsigned long long t_begin = s.instrument ? wall_clock64() : 0, busy = 0;
+    int reason = kExitDone;
+
+    while (true) {
+        unsigned n = frontier_length(s, c.step);
+        if (ld_agent(&s.ctl->overflow)) { reason = kExitOverflow; break; }
+        if (n == 0) { reason = kExitDone; break; }
+        if (loop == kLoopCE && s.sat_t >= 0 && n > (unsigned) s.sat_t) {
+            reason = kExitSaturated;
+            break;
+        }
+
+        if (s.fast_t >= 0 && n <= (unsigned) s.fast_t && gridDim.x > 1) {
+            // Single-workgroup fast path: block 0 runs consecutive small
+            // levels with workgroup barriers only (no grid tier: it alone
+            // works); the rest of the grid waits at the next grid barrier.
+            if (blockIdx.x == 0) {
+                unsigned long long t0 = s.instrument ? wall_clock64() : 0;
+                while (true) {
+                    if (threadIdx.x == 0) prepare_step(s.ctl, c, n);
+                    run_body<PRED, EXACT>(c.phase, g, s, c, n, lds, 0, false);
+                    __syncthreads();
+                    c = advance(loop, c);
+                    if (threadIdx.x == 0) lds.nb = frontier_length(s, c.step);
+                    __syncthreads();
+                    n = lds.nb;
+                    if (n == 0 || n > (unsigned) s.fast_t || ld_agent(&s.ctl->overflow)) break;
+                    if (loop == kLoopCE && s.sat_t >= 0 && n > (unsigned) s.sat_t) break;
+                }
+                if (threadIdx.x == 0) {
+                    st_agent(&s.ctl->fast_step, c.step);
+                    st_agent(&s.ctl->fast_iteration, c.iteration);
+                    st_agent(&s.ctl->fast_phase, c.phase);
+                }
+                if (s.instrument) busy += wall_clock64() - t0;
+            }
+            grid_barrier(s.ctl);
+            c.step = ld_agent(&s.ctl->fast_step);
+            c.iteration = ld_agent(&s.ctl->fast_iteration);
+            c.phase = ld_agent(&s.ctl->fast_phase);
+            continue;
+        }
+
+        unsigned long long t0 = s.instrument ? wall_clock64() : 0;
+        if (blockIdx.x == 0 && threadIdx.x == 0) prepare_step(s.ctl, c, n);
+        run_body<PRED, EXACT>(c.phase, g, s, c, n, lds, 0, true);
+        if (s.instrument) busy += wall_clock64() - t0;
+        grid_barrier(s.ctl);
+        if (gathers(c.phase) && hubs_pending(s, c.step)) {
+            t0 = s.instrument ? wall_clock64() : 0;
+            run_body<PRED, EXACT>(c.phase, g, s, c, n, lds, 1, true);
+            if (s.instrument) busy += wall_clock64() - t0;
+            grid_barrier(s.ctl);
+        }
+        c = advance(loop, c);
+    }
+
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        s.ctl->exit_reason = reason;
+        s.ctl->exit_step = c.step;
+        s.ctl->exit_iteration = c.iteration;
+        s.ctl->exit_phase = c.phase;
+    }
+    if (s.instrument && threadIdx.x == 0) {
+        unsigned long long life = wall_clock64() - t_begin;
+        atomicAdd(&s.ctl->busy_cycles, busy);
+        atomicAdd(&s.ctl->life_cycles, life);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launches per step: the tile phase, then (for gathering phases) the hub
+// phase as its own launch.
+// ---------------------------------------------------------------------------
+template <bool PRED, bool EXACT, int PHASE>
+__global__ __launch_bounds__(kBlock) void step_kernel(GraphArgs g, SearchArgs s, Cursor c)
+{
+    __shared__ Lds<PRED> lds;
+    unsigned n = frontier_length(s, c.step);
+    if (ld_agent(&s.ctl->overflow)) n = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) prepare_step(s.ctl, c, n);
+    if (n == 0) return;
+    init_lds(lds);
+    run_body<PRED, EXACT>(PHASE, g, s, c, n, lds, 0, true);
+}
+
+template <bool PRED, bool EXACT, int PHASE>
+__global__ __launch_bounds__(kBlock) void hub_kernel(GraphArgs g, SearchArgs s, Cursor c)
+{
+    __shared__ Lds<PRED> lds;
+    if (!hubs_pending(s, c.step)) return;
+    init_lds(lds);
+    run_body<PRED, EXACT>(PHASE, g, s, c, 0, lds, 1, true);
+}
+
+// ---------------------------------------------------------------------------
+// Initialisation kernels.
+// ---------------------------------------------------------------------------
+__global__ void reset_labels_kernel(int *depth, int *pred, unsigned *visited, int nodes)
+{
+    int words = (nodes + 31) >> 5;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nodes; i += gridDim.x * blockDim.x) {
+        depth[i] = -1;
+        if (pred) pred[i] = -1;
+        if (i < words) visited[i] = 0u;
+    }
+}
+
+// Seeds the search: counters to zero, the source in queue 0 (predecessor -2),
+// and for vertex-frontier strategies the source already labelled.  Resetting
+// every counter here, before the first step, replaces the reference's
+// iteration-0 counter reset inside every CTA (contract_expand_atomic/
+// kernel.cuh:307-332), which can race with CTA 0's first enqueue.
+__global__ void init_search_kernel(SearchArgs s, int src, int label_source)
+{
+    Control *ctl = s.ctl;
+    unsigned *words = reinterpret_cast<unsigned *>(ctl);
+    for (int i = threadIdx.x; i < (int) (sizeof(Control) / 4); i += blockDim.x) words[i] = 0u;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ctl->qlen[0] = 1;
+        ctl->max_depth = -1;
+        s.q[0][0] = src;
+        if (s.pq[0]) s.pq[0][0] = -2;
+        if (label_source) {
+            s.depth[src] = 0;
+            if (s.pred) s.pred[src] = -2;
+            s.visited[src >> 5] |= 1u << (src & 31);
+            ctl->max_depth = 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Barrier microbenchmark and primitive self-test.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void barrier_bench_kernel(Control *ctl, int iterations)
+{
+    for (int i = 0; i < iterations; i++) grid_barrier(ctl);
+}
+
+}  // namespace b40c_hip
+
+// Plain 16-byte-vector streaming copy (counter calibration).
+extern "C" __global__ __launch_bounds__(256) void b40c_calibration_copy_kernel(const uint4 *src, uint4 *dst,
+                                                                              size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t) blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+namespace b40c_hip {
+
+__global__ __launch_bounds__(kWave) void wave_selftest_kernel(const int *in, int *out_scan, int *out_rank, int n)
+{
+    int i = blockIdx.x * kWave + threadIdx.x;
+    int x = i < n ? in[i] : 0;
+    int s = wave_inclusive_sum(x);
+    uint64_t m = ballot((x & 1) != 0);
+    int r = lanes_below(m);
+    if (i < n) {
+        out_scan[i] = s;
+        out_rank[i] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+namespace {
+
+template <bool PRED, bool EXACT>
+const void *persistent_ptr()
+{
+    return reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT>);
+}
+
+const void *persistent_select(bool pred, bool exact)
+{
+    if (pred) return exact ? persistent_ptr<true, true>() : persistent_ptr<true, false>();
+    return exact ? persistent_ptr<false, true>() : persistent_ptr<false, false>();
+}
+
+template <bool PRED, bool EXACT, int PHASE>
+hipError_t launch_phase(const GraphArgs &g, const SearchArgs &s, Cursor c, int grid, hipStream_t st, int *launches)
+{
+    hipLaunchKernelGGL((step_kernel<PRED, EXACT, PHASE>), dim3(grid), dim3(kBlock), 0, st, g, s, c);
+    (*launches)++;
+    if (PHASE == kPhaseExpandContract || PHASE == kPhaseContractExpand || PHASE == kPhaseExpand) {
+        hipLaunchKernelGGL((hub_kernel<PRED, EXACT, PHASE>), dim3(grid), dim3(kBlock), 0, st, g, s, c);
+        (*launches)++;
+    }
+    return hipGetLastError();
+}
+
+template <bool PRED, bool EXACT>
+hipError_t launch_step_t(int phase, const GraphArgs &g, const SearchArgs &s, Cursor c, int grid, hipStream_t st,
+                         int *launches)
+{
+    switch (phase) {
+    case kPhaseExpandContract: return launch_phase<PRED, EXACT, kPhaseExpandContract>(g, s, c, grid, st, launches);
+    case kPhaseContractExpand: return launch_phase<PRED, EXACT, kPhaseContractExpand>(g, s, c, grid, st, launches);
+    case kPhaseContract: return launch_phase<PRED, EXACT, kPhaseContract>(g, s, c, grid, st, launches);
+    case kPhaseExpand: return launch_phase<PRED, EXACT, kPhaseExpand>(g, s, c, grid, st, launches);
+    case kPhaseFilter: return launch_phase<PRED, EXACT, kPhaseFilter>(g, s, c, grid, st, launches);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_reset_labels(int *depth, int *pred, unsigned *visited, int nodes, hipStream_t st)
+{
+    int grid = (nodes + 255) / 256;
+    if (grid > 16384) grid = 16384;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(reset_labels_kernel, dim3(grid), dim3(256), 0, st, depth, pred, visited, nodes);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_search(const SearchArgs &s, int src, int nodes, int label_source, hipStream_t st)
+{
+    (void) nodes;
+    hipLaunchKernelGGL(init_search_kernel, dim3(1), dim3(256), 0, st, s, src, label_source);
+    return hipGetLastError();
+}
+
+hipError_t launch_step(int phase, bool pred, bool exact, const GraphArgs &g, const SearchArgs &s, Cursor c,
+                       int grid, hipStream_t st, int *launches)
+{
+    if (pred) return exact ? launch_step_t<true, true>(phase, g, s, c, grid, st, launches)
+                           : launch_step_t<true, false>(phase, g, s, c, grid, st, launches);
+    return exact ? launch_step_t<false, true>(phase, g, s, c, grid, st, launches)
+                 : launch_step_t<false, false>(phase, g, s, c, grid, st, launches);
+}
+
+hipError_t launch_persistent(int loop, bool pred, bool exact, const GraphArgs &g, const SearchArgs &s, Cursor c,
+                             int grid, hipStream_t st)
+{
+    GraphArgs ga = g;
+    SearchArgs sa = s;
+    Cursor ca = c;
+    int la = loop;
+    void *args[] = {&ga, &sa, &ca, &la};
+    return hipLaunchCooperativeKernel(persistent_select(pred, exact), dim3(grid), dim3(kBlock), args, 0, st);
+}
+
+int persistent_max_blocks_per_cu(int loop, bool pred, bool exact)
+{
+    (void) loop;
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, persistent_select(pred, exact), kBlock, 0) != hipSuccess)
+        return 0;
+    return blocks;
+}
+
+int step_max_blocks_per_cu(bool pred)
+{
+    int blocks = 0;
+    const void *k = pred ? reinterpret_cast<const void *>(&step_kernel<true, true, kPhaseExpandContract>)
+                         : reinterpret_cast<const void *>(&step_kernel<false, true, kPhaseExpandContract>);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, kBlock, 0) != hipSuccess) return 0;
+    return blocks;
+}
+
+hipError_t launch_barrier_bench(Control *ctl, int iterations, int grid, hipStream_t st)
+{
+    Control *c = ctl;
+    int it = iterations;
+    void *args[] = {&c, &it};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&barrier_bench_kernel), dim3(grid),
+                                      dim3(kBlock), args, 0, st);
+}
+
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
+{
+    size_t n = bytes / 16;
+    hipLaunchKernelGGL(b40c_calibration_copy_kernel, dim3(2048), dim3(256), 0, st,
+                       reinterpret_cast<const uint4 *>(src), reinterpret_cast<uint4 *>(dst), n);
+    return hipGetLastError();
+}
+
+hipError_t launch_wave_selftest(const int *in, int *out_scan, int *out_rank, int n, hipStream_t st)
+{
+    int grid = (n + kWave - 1) / kWave;
+    hipLaunchKernelGGL(wave_selftest_kernel, dim3(grid), dim3(kWave), 0, st, in, out_scan, out_rank, n);
+    return hipGetLastError();
+}
+
+}  // namespace b40c_hip

@@ -1,0 +1,233 @@
+"""GPU parity: every strategy's depth labels are bit-exact against the CPU oracle.
+
+All calls go through the C ABI (libb40c_hip.so).  Predecessors are checked by
+the invariant of test/bfs/test_bfs.cu:387-433 (tightened), since racing
+discoverers may legitimately pick different parents.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import back40computing_amd as b40c
+from back40computing_amd import CsrGraph, CsrProblem
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "tiny_graphs.json")
+
+# (strategy, fused) pairs: the reference's four enactors, persistent and per-level.
+VARIANTS = [
+    (b40c.EXPAND_CONTRACT, True), (b40c.EXPAND_CONTRACT, False),
+    (b40c.CONTRACT_EXPAND, True), (b40c.CONTRACT_EXPAND, False),
+    (b40c.TWO_PHASE, True), (b40c.TWO_PHASE, False),
+    (b40c.HYBRID, True),
+]
+VARIANT_IDS = ["ec_fused", "ec_iter", "ce_fused", "ce_iter", "tp_fused", "tp_iter", "hybrid"]
+
+
+def frontier_type(strategy):
+    return b40c.VERTEX_FRONTIERS if strategy == b40c.EXPAND_CONTRACT else b40c.EDGE_FRONTIERS
+
+
+def run(problem, strategy, fused, src, queue_sizing=1.3, max_grid_size=0):
+    problem.reset(frontier_type(strategy), queue_sizing)
+    stats = problem.enact(strategy, int(src), fused=fused, max_grid_size=max_grid_size)
+    return problem.extract_depths(), stats
+
+
+def assert_parity(g, problem, strategy, fused, src, **kw):
+    row, col = g.row_offsets, g.column_indices
+    expect, levels, _ = oracle.bfs(row, col, int(src))
+    got, stats = run(problem, strategy, fused, src, **kw)
+    bad = oracle.compare(got, expect)
+    assert bad == -1, "vertex %d: got %d expected %d (strategy %d fused %d src %d)" % (
+        bad, got[bad], expect[bad], strategy, fused, src)
+    assert stats.search_depth == levels
+    if problem.mark_predecessors:
+        preds = problem.extract_results()
+        assert oracle.check_preds(row, col, expect, preds) == -1
+    return stats
+
+
+def test_wave_primitives_selftest(gpu):
+    rc, bad = b40c.selftest_wave_primitives()
+    assert rc == 0 and bad == 0, bad
+
+
+def test_grid_barrier_runs(gpu):
+    us, grid = b40c.barrier_benchmark(blocks_per_cu=2, iterations=2000)
+    assert grid > 0 and 0 < us < 1000
+
+
+@pytest.mark.parametrize("variant", range(len(VARIANTS)), ids=VARIANT_IDS)
+@pytest.mark.parametrize("mark_pred", [False, True], ids=["depth", "pred"])
+def test_golden_graphs(gpu, variant, mark_pred):
+    strategy, fused = VARIANTS[variant]
+    with open(GOLDEN) as f:
+        graphs = json.load(f)["graphs"]
+    for gd in graphs:
+        g = CsrGraph.from_arrays(gd["row_offsets"], gd["column_indices"])
+        p = CsrProblem(mark_predecessors=mark_pred).from_host_problem(g)
+        for src, expected, _ in gd["cases"]:
+            got, _ = run(p, strategy, fused, src)
+            assert np.array_equal(got, np.array(expected, dtype=np.int32)), (gd["name"], src)
+            if mark_pred:
+                assert oracle.check_preds(g.row_offsets, g.column_indices, got, p.extract_results()) == -1
+        p.destroy()
+
+
+@pytest.mark.parametrize("variant", range(len(VARIANTS)), ids=VARIANT_IDS)
+def test_random_multigraphs(gpu, variant):
+    strategy, fused = VARIANTS[variant]
+    rng = np.random.default_rng(1234 + variant)
+    for trial in range(6):
+        n = int(rng.integers(2, 3000))
+        m = int(rng.integers(0, 6 * n))
+        g = CsrGraph.from_coo(n, rng.integers(0, n, m), rng.integers(0, n, m),
+                              symmetrize=bool(trial % 2), dedup=bool(trial % 3 == 0))
+        p = CsrProblem(mark_predecessors=bool(trial % 2)).from_host_problem(g)
+        for src in rng.integers(0, n, 3):
+            assert_parity(g, p, strategy, fused, src)
+        p.destroy()
+
+
+@pytest.mark.parametrize("variant", range(len(VARIANTS)), ids=VARIANT_IDS)
+@pytest.mark.parametrize("cull", [b40c.CULL_EXACT, b40c.CULL_BEST_EFFORT], ids=["exact", "best_effort"])
+def test_rmat_scale14(gpu, variant, cull):
+    strategy, fused = VARIANTS[variant]
+    g = CsrGraph.rmat(14, 16, seed=0xAB)
+    p = CsrProblem(mark_predecessors=(variant % 2 == 0)).from_host_problem(g)
+    p.set_config(cull_mode=cull)
+    # The best-effort discipline lets one duplicate per XCD through a level
+    # (plain stores stay in that XCD's write-back L2 until the level
+    # barrier), so edge-frontier strategies need more queue headroom.
+    qs = 2.0 if cull == b40c.CULL_EXACT else 64.0
+    for src in g.sample_sources(3, seed=variant):
+        assert_parity(g, p, strategy, fused, src, queue_sizing=qs)
+    p.destroy()
+
+
+@pytest.mark.parametrize("variant", range(len(VARIANTS)), ids=VARIANT_IDS)
+def test_gather_tiers_and_grid_shapes(gpu, variant):
+    """Small tier thresholds push short rows through the CTA and wave tiers;
+    tiny grids and a disabled fast path exercise the barrier loop."""
+    strategy, fused = VARIANTS[variant]
+    g = CsrGraph.rmat(12, 8, seed=3)
+    p = CsrProblem().from_host_problem(g)
+    src = int(g.sample_sources(1, seed=9)[0])
+    for wave_t, cta_t, grid_t, fast_t, grid in [(1, 2, 3, -1, 0), (2, 5, 40, 64, 3), (64, 65, 100, -1, 1),
+                                                (1000000, 1000000, -1, 16, 0), (8, 4096, 2049, 1 << 30, 0),
+                                                (64, 1024, 8192, 1024, 0)]:
+        p.set_config(wave_gather_threshold=wave_t, cta_gather_threshold=cta_t,
+                     grid_gather_threshold=grid_t, fast_path_threshold=fast_t)
+        assert_parity(g, p, strategy, fused, src, max_grid_size=grid)
+    p.destroy()
+
+
+def test_hybrid_switches_back_and_forth(gpu):
+    g = CsrGraph.rmat(15, 16, seed=21)
+    p = CsrProblem(mark_predecessors=True).from_host_problem(g)
+    for sat in [1, 50, 2000, -1]:
+        p.set_config(saturation_threshold=sat)
+        for src in g.sample_sources(2, seed=sat & 0xFF):
+            st = assert_parity(g, p, b40c.HYBRID, True, src)
+            if sat == 1:
+                assert st.kernel_launches > 1
+    p.destroy()
+
+
+def test_grid_lattices_small(gpu):
+    for w, builder in [(37, CsrGraph.grid2d), (11, CsrGraph.grid3d)]:
+        g = builder(w)
+        p = CsrProblem().from_host_problem(g)
+        for strategy, fused in VARIANTS:
+            assert_parity(g, p, strategy, fused, 0)
+            assert_parity(g, p, strategy, fused, g.nodes // 2)
+        p.destroy()
+
+
+def test_overflow_is_reported_and_recoverable(gpu):
+    g = CsrGraph.rmat(12, 16, seed=5)
+    p = CsrProblem().from_host_problem(g)
+    src = int(g.sample_sources(1, seed=2)[0])
+    p.set_config(cull_mode=b40c.CULL_BEST_EFFORT)
+    for strategy, fused in [(b40c.CONTRACT_EXPAND, True), (b40c.TWO_PHASE, False), (b40c.HYBRID, True)]:
+        p.reset(b40c.EDGE_FRONTIERS, 0.001)
+        with pytest.raises(b40c.BfsError) as ei:
+            p.enact(strategy, src, fused=fused)
+        assert ei.value.status == 3  # BFS_ERR_OVERFLOW
+    p.set_config(cull_mode=b40c.CULL_EXACT)
+    assert_parity(g, p, b40c.HYBRID, True, src)
+    p.destroy()
+
+
+def test_invalid_source_and_frontier_type(gpu):
+    g = CsrGraph.grid2d(4)
+    p = CsrProblem().from_host_problem(g)
+    p.reset(b40c.EDGE_FRONTIERS, 1.3)
+    with pytest.raises(b40c.BfsError):
+        p.enact(b40c.HYBRID, 16)
+    with pytest.raises(b40c.BfsError):
+        p.enact(b40c.EXPAND_CONTRACT, 0)  # needs vertex frontiers
+    p.destroy()
+
+
+def test_repeated_enacts_without_reset(gpu):
+    g = CsrGraph.rmat(13, 8, seed=77)
+    p = CsrProblem().from_host_problem(g)
+    p.reset(b40c.EDGE_FRONTIERS, 1.3)
+    row, col = g.row_offsets, g.column_indices
+    for src in g.sample_sources(4, seed=4):
+        p.enact(b40c.HYBRID, int(src))
+        expect, _, _ = oracle.bfs(row, col, int(src))
+        assert oracle.compare(p.extract_depths(), expect) == -1
+    p.destroy()
+
+
+# ---------------------------------------------------------------------------
+# Full-size configurations
+# ---------------------------------------------------------------------------
+@pytest.mark.slow
+def test_config1_grid2d_1000(gpu):
+    g = CsrGraph.grid2d(1000)
+    assert g.nodes == 1_000_000 and g.edges == 3_996_000
+    j, k = np.divmod(np.arange(g.nodes), 1000)
+    expect = (j + k).astype(np.int32)
+    p = CsrProblem().from_host_problem(g)
+    for strategy, fused in VARIANTS:
+        got, st = run(p, strategy, fused, 0)
+        assert oracle.compare(got, expect) == -1, (strategy, fused)
+        assert st.search_depth == 1999
+    p.destroy()
+
+
+@pytest.mark.slow
+def test_config2_rmat22(gpu):
+    g = CsrGraph.rmat(22, 16)
+    p = CsrProblem().from_host_problem(g)
+    srcs = g.sample_sources(64)
+    for i, (strategy, fused) in enumerate([(b40c.EXPAND_CONTRACT, True), (b40c.HYBRID, True),
+                                           (b40c.TWO_PHASE, False), (b40c.CONTRACT_EXPAND, True)]):
+        assert_parity(g, p, strategy, fused, srcs[i])
+    p.destroy()
+
+
+@pytest.mark.slow
+def test_config3_grid3d_300(gpu):
+    w = 300
+    g = CsrGraph.grid3d(w)
+    assert g.nodes == 27_000_000 and g.edges == 161_460_000
+    idx = np.arange(g.nodes, dtype=np.int64)
+    i, rem = np.divmod(idx, w * w)
+    jj, kk = np.divmod(rem, w)
+    expect = (i + jj + kk).astype(np.int32)
+    del idx, i, rem, jj, kk
+    p = CsrProblem().from_host_problem(g)
+    for strategy, fused in [(b40c.EXPAND_CONTRACT, True), (b40c.HYBRID, True)]:
+        got, st = run(p, strategy, fused, 0)
+        assert oracle.compare(got, expect) == -1
+        assert st.search_depth == 898
+    p.destroy()
