@@ -33,7 +33,9 @@ class BfsConfig(ctypes.Structure):
         ("cull_mode", ctypes.c_int32),
         ("instrument", ctypes.c_int32),
         ("grid_gather_threshold", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 8),
+        ("do_alpha", ctypes.c_int32),
+        ("do_beta", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 

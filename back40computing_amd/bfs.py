@@ -22,7 +22,7 @@ import numpy as np
 
 from . import _native
 
-EXPAND_CONTRACT, CONTRACT_EXPAND, TWO_PHASE, HYBRID = 0, 1, 2, 3
+EXPAND_CONTRACT, CONTRACT_EXPAND, TWO_PHASE, HYBRID, DIRECTION_OPTIMIZING = 0, 1, 2, 3, 4
 VERTEX_FRONTIERS, EDGE_FRONTIERS, MIXED_FRONTIERS = 0, 1, 2
 CULL_EXACT, CULL_BEST_EFFORT = 0, 1
 
@@ -276,6 +276,11 @@ class EnactorContractExpand(EnactorExpandContract):
 
 class EnactorTwoPhase(EnactorExpandContract):
     STRATEGY = TWO_PHASE
+
+
+class EnactorDirectionOptimizing(EnactorExpandContract):
+    """Addition: top-down expand-contract levels mixed with bottom-up levels."""
+    STRATEGY = DIRECTION_OPTIMIZING
 
 
 class EnactorHybrid(_Enactor):

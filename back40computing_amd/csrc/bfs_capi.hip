@@ -13,7 +13,10 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
+
+#include "b40c_graph.h"
 
 using namespace b40c_hip;
 
@@ -50,7 +53,9 @@ const bfs_config kDefaults = {
     /*cull_mode*/ BFS_CULL_EXACT,
     /*instrument*/ 0,
     /*grid_gather_threshold*/ 8192,
-    {0, 0, 0, 0, 0, 0, 0, 0}};
+    /*do_alpha*/ 14,
+    /*do_beta*/ 24,
+    {0, 0, 0, 0, 0, 0}};
 
 constexpr unsigned kHubCapacity = 1u << 16;  // grid-tier entries per step list
 
@@ -67,6 +72,9 @@ struct bfs_problem {
 
     int *d_row = nullptr;
     int *d_col = nullptr;
+    int *d_rrow = nullptr;       // reverse CSR (aliases d_row / d_col when symmetric)
+    int *d_rcol = nullptr;
+    bool symmetric = true;
     int *d_depth = nullptr;
     int *d_pred = nullptr;
     unsigned *d_visited = nullptr;
@@ -80,6 +88,7 @@ struct bfs_problem {
     Control *d_ctl = nullptr;
     Control *h_ctl = nullptr;    // pinned mirror
     HubEntry *d_hubs = nullptr;  // two grid-tier lists of kHubCapacity entries
+    unsigned *d_fbm = nullptr;   // two frontier bitmaps (bottom-up levels)
     int wall_khz = 100000;
     bfs_config cfg = kDefaults;
 };
@@ -133,6 +142,14 @@ SearchArgs search_args(const bfs_problem *p, int grid)
     s.sat_t = p->cfg.saturation_threshold > 0 ? p->cfg.saturation_threshold
               : (p->cfg.saturation_threshold < 0 ? -1 : grid * 256);
     s.instrument = p->cfg.instrument;
+    size_t words = ((size_t) p->nodes + 31) / 32;
+    s.fbm[0] = p->d_fbm;
+    s.fbm[1] = p->d_fbm + words;
+    s.nodes = p->nodes;
+    s.total_edges = (unsigned long long) p->edges;
+    s.track_edges = 1;
+    s.do_alpha = p->cfg.do_alpha;
+    s.do_beta = p->cfg.do_beta;
     return s;
 }
 
@@ -224,6 +241,81 @@ bfs_status enact_hybrid(bfs_problem *p, const GraphArgs &g, const SearchArgs &s,
     }
 }
 
+// Per-level launches of the direction-optimizing search: the host reads
+// the frontier counters after every level and applies the same switch rule
+// as the persistent kernel.
+bfs_status enact_direction_iterative(bfs_problem *p, const GraphArgs &g, const SearchArgs &s, int grid,
+                                     int *launches)
+{
+    const bool exact = p->cfg.cull_mode == BFS_CULL_EXACT;
+    Cursor c{0, 0, kPhaseExpandContract};
+    while (true) {
+        bfs_status st = read_control(p);
+        if (st != BFS_OK) return st;
+        const Control &h = *p->h_ctl;
+        unsigned n = h.qlen[c.step % 3];
+        if (h.overflow || n == 0) return BFS_OK;
+        unsigned long long mf = h.fedges[c.step % 3];
+        unsigned long long mu = s.total_edges > h.visited_edges ? s.total_edges - h.visited_edges : 0ull;
+        int dir = c.phase;
+        if (c.phase == kPhaseExpandContract) {
+            if (mf * (unsigned long long) s.do_alpha > mu) dir = kPhaseBottomUp;
+        } else if ((unsigned long long) n * (unsigned long long) s.do_beta < (unsigned long long) s.nodes) {
+            dir = kPhaseExpandContract;
+        }
+        if (dir == kPhaseBottomUp && c.phase != kPhaseBottomUp)
+            HIP_TRY(launch_build_frontier_bitmap(s, c, grid, p->stream, launches), "build frontier bitmap");
+        c.phase = dir;
+        HIP_TRY(launch_step(c.phase, p->mark_pred, exact, g, s, c, grid, p->stream, launches), "launch step");
+        c.step++;
+        c.iteration++;
+    }
+}
+
+// Whether the edge multiset equals its reverse, by two independent 64-bit
+// commutative hashes of (u, v) and (v, u) over all edges.  Unequal sums prove
+// asymmetry; equal sums are taken as symmetry (a false positive needs a
+// 128-bit collision).
+bool csr_is_symmetric(const int32_t *row, const int32_t *col, int32_t nodes)
+{
+    auto mix = [](uint64_t z) {
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    int threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<uint64_t> acc(4 * threads, 0);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; t++) {
+        pool.emplace_back([&, t] {
+            uint64_t a = 0, b = 0, c = 0, d = 0;
+            for (int32_t u = t; u < nodes; u += threads) {
+                for (int32_t e = row[u]; e < row[u + 1]; e++) {
+                    uint64_t v = (uint32_t) col[e];
+                    uint64_t fw = ((uint64_t) (uint32_t) u << 32) | v, bw = (v << 32) | (uint32_t) u;
+                    a += mix(fw);
+                    b += mix(bw);
+                    c += mix(fw ^ 0x5851F42D4C957F2Dull);
+                    d += mix(bw ^ 0x5851F42D4C957F2Dull);
+                }
+            }
+            acc[4 * t] = a;
+            acc[4 * t + 1] = b;
+            acc[4 * t + 2] = c;
+            acc[4 * t + 3] = d;
+        });
+    }
+    for (auto &th : pool) th.join();
+    uint64_t a = 0, b = 0, c = 0, d = 0;
+    for (int t = 0; t < threads; t++) {
+        a += acc[4 * t];
+        b += acc[4 * t + 1];
+        c += acc[4 * t + 2];
+        d += acc[4 * t + 3];
+    }
+    return a == b && c == d;
+}
+
 bfs_status validate_csr(const int32_t *row, const int32_t *col, int32_t nodes, int64_t edges)
 {
     if (!row || (edges > 0 && !col) || nodes <= 0 || edges < 0 || edges > INT32_MAX)
@@ -272,7 +364,7 @@ const char *bfs_last_error_message(void) { return g_last_error.c_str(); }
 
 bfs_frontier_type bfs_strategy_frontier_type(bfs_strategy s)
 {
-    return s == BFS_EXPAND_CONTRACT ? BFS_VERTEX_FRONTIERS : BFS_EDGE_FRONTIERS;
+    return (s == BFS_EXPAND_CONTRACT || s == BFS_DIRECTION_OPTIMIZING) ? BFS_VERTEX_FRONTIERS : BFS_EDGE_FRONTIERS;
 }
 
 bfs_status bfs_config_default(bfs_config *cfg)
@@ -335,7 +427,8 @@ bfs_status bfs_problem_create(const int32_t *h_row_offsets, const int32_t *h_col
         (e = hipMalloc(&p->d_depth, sizeof(int) * (size_t) nodes)) != hipSuccess ||
         (e = hipMalloc(&p->d_visited, sizeof(unsigned) * words)) != hipSuccess ||
         (e = hipMalloc(&p->d_ctl, sizeof(Control))) != hipSuccess ||
-        (e = hipMalloc(&p->d_hubs, sizeof(HubEntry) * 2 * kHubCapacity)) != hipSuccess)
+        (e = hipMalloc(&p->d_hubs, sizeof(HubEntry) * 2 * kHubCapacity)) != hipSuccess ||
+        (e = hipMalloc(&p->d_fbm, sizeof(unsigned) * 2 * words)) != hipSuccess)
         return cleanup(hip_fail(e, "allocate problem"));
     if (p->mark_pred && (e = hipMalloc(&p->d_pred, sizeof(int) * (size_t) nodes)) != hipSuccess)
         return cleanup(hip_fail(e, "allocate predecessors"));
@@ -346,6 +439,29 @@ bfs_status bfs_problem_create(const int32_t *h_row_offsets, const int32_t *h_col
     if (edges > 0 &&
         (e = hipMemcpy(p->d_col, h_column_indices, sizeof(int) * (size_t) edges, hipMemcpyHostToDevice)) != hipSuccess)
         return cleanup(hip_fail(e, "copy column indices"));
+
+    // Bottom-up levels scan in-neighbours: reuse the CSR when the graph is
+    // symmetric, otherwise upload its transpose.
+    p->symmetric = csr_is_symmetric(h_row_offsets, h_column_indices, nodes);
+    if (p->symmetric) {
+        p->d_rrow = p->d_row;
+        p->d_rcol = p->d_col;
+    } else {
+        std::vector<int32_t> src((size_t) edges);
+        for (int32_t v = 0; v < nodes; v++)
+            for (int32_t k = h_row_offsets[v]; k < h_row_offsets[v + 1]; k++) src[k] = v;
+        b40c_csr t{};
+        if (b40c_graph_from_coo(nodes, edges, h_column_indices, src.data(), 0, 0, 0, &t) != 0)
+            return cleanup(fail(BFS_ERR_OOM, "transpose construction failed"));
+        bool ok = hipMalloc(&p->d_rrow, sizeof(int) * ((size_t) nodes + 1)) == hipSuccess &&
+                  hipMalloc(&p->d_rcol, sizeof(int) * (size_t) std::max<int64_t>(edges, 1)) == hipSuccess &&
+                  hipMemcpy(p->d_rrow, t.row_offsets, sizeof(int) * ((size_t) nodes + 1), hipMemcpyHostToDevice) ==
+                      hipSuccess &&
+                  (edges == 0 || hipMemcpy(p->d_rcol, t.column_indices, sizeof(int) * (size_t) edges,
+                                           hipMemcpyHostToDevice) == hipSuccess);
+        b40c_graph_free(&t);
+        if (!ok) return cleanup(fail(BFS_ERR_OOM, "allocate or copy the transpose"));
+    }
     *out = p;
     return BFS_OK;
 }
@@ -387,6 +503,9 @@ bfs_status bfs_problem_set_config(bfs_problem *p, const bfs_config *cfg)
     if (c.blocks_per_cu == 0) c.blocks_per_cu = kDefaults.blocks_per_cu;
     if (c.grid_gather_threshold == 0) c.grid_gather_threshold = kDefaults.grid_gather_threshold;
     if (c.grid_gather_threshold < 0) c.grid_gather_threshold = -1;  // grid tier off
+    if (c.do_alpha == 0) c.do_alpha = kDefaults.do_alpha;
+    if (c.do_beta == 0) c.do_beta = kDefaults.do_beta;
+    if (c.do_alpha < 1 || c.do_beta < 1) return fail(BFS_ERR_INVALID_ARG, "invalid direction switch points");
     if (c.wave_gather_threshold < 1 || c.cta_gather_threshold < 1 || c.blocks_per_cu < 1 ||
         (c.cull_mode != BFS_CULL_EXACT && c.cull_mode != BFS_CULL_BEST_EFFORT))
         return fail(BFS_ERR_INVALID_ARG, "invalid config");
@@ -406,7 +525,8 @@ bfs_status bfs_enact(bfs_problem *p, bfs_strategy strategy, int32_t fused, int32
                      bfs_stats *stats)
 {
     if (!p) return fail(BFS_ERR_INVALID_ARG, "null problem");
-    if (strategy < BFS_EXPAND_CONTRACT || strategy > BFS_HYBRID) return fail(BFS_ERR_INVALID_ARG, "unknown strategy");
+    if (strategy < BFS_EXPAND_CONTRACT || strategy > BFS_DIRECTION_OPTIMIZING)
+        return fail(BFS_ERR_INVALID_ARG, "unknown strategy");
     if (src < 0 || src >= p->nodes) return fail(BFS_ERR_INVALID_ARG, "source vertex out of range");
     if (p->frontier_type < 0) return fail(BFS_ERR_INVALID_ARG, "problem not reset");
     bfs_frontier_type need = bfs_strategy_frontier_type(strategy);
@@ -419,17 +539,22 @@ bfs_status bfs_enact(bfs_problem *p, bfs_strategy strategy, int32_t fused, int32
     p->dirty = true;
 
     const bool exact = p->cfg.cull_mode == BFS_CULL_EXACT;
-    int loop = strategy == BFS_EXPAND_CONTRACT ? kLoopEC : (strategy == BFS_TWO_PHASE ? kLoopTP : kLoopCE);
+    int loop = strategy == BFS_EXPAND_CONTRACT ? kLoopEC
+               : strategy == BFS_TWO_PHASE       ? kLoopTP
+               : strategy == BFS_DIRECTION_OPTIMIZING ? kLoopDO
+                                                 : kLoopCE;
+    const bool vertex_frontier = strategy == BFS_EXPAND_CONTRACT || strategy == BFS_DIRECTION_OPTIMIZING;
     int pgrid = persistent_grid(p, loop, max_grid_size);
     if (pgrid > p->num_cus * std::max(1, persistent_max_blocks_per_cu(loop, p->mark_pred, exact)))
         return fail(BFS_ERR_LAUNCH, "grid exceeds co-resident capacity");
     int sgrid = step_grid(p, max_grid_size);
-    GraphArgs g{p->d_row, p->d_col, p->nodes};
+    GraphArgs g{p->d_row, p->d_col, p->nodes, p->d_rrow, p->d_rcol};
     SearchArgs s = search_args(p, pgrid);
     if (strategy != BFS_HYBRID) s.sat_t = -1;
 
     int launches = 0;
-    HIP_TRY(launch_init_search(s, src, p->nodes, strategy == BFS_EXPAND_CONTRACT ? 1 : 0, p->stream), "init search");
+    if (!vertex_frontier) s.track_edges = 0;
+    HIP_TRY(launch_init_search(s, p->d_row, src, vertex_frontier ? 1 : 0, p->stream), "init search");
     HIP_TRY(hipEventRecord(p->ev_start, p->stream), "record start");
     bfs_status st = BFS_OK;
     Cursor c{0, 0, kPhaseExpandContract};
@@ -463,6 +588,15 @@ bfs_status bfs_enact(bfs_problem *p, bfs_strategy strategy, int32_t fused, int32
         break;
     case BFS_HYBRID:
         st = enact_hybrid(p, g, s, pgrid, sgrid, &launches);
+        break;
+    case BFS_DIRECTION_OPTIMIZING:
+        c.phase = kPhaseExpandContract;
+        if (fused) {
+            HIP_TRY(launch_persistent(kLoopDO, p->mark_pred, exact, g, s, c, pgrid, p->stream), "launch persistent");
+            launches++;
+        } else {
+            st = enact_direction_iterative(p, g, s, sgrid, &launches);
+        }
         break;
     }
     if (st != BFS_OK) return st;
@@ -507,12 +641,15 @@ void bfs_problem_destroy(bfs_problem *p)
     (void) hipSetDevice(p->device);
     if (p->stream) (void) hipStreamSynchronize(p->stream);
     free_queues(p);
+    if (p->d_rrow && p->d_rrow != p->d_row) (void) hipFree(p->d_rrow);
+    if (p->d_rcol && p->d_rcol != p->d_col) (void) hipFree(p->d_rcol);
     int *bufs[] = {p->d_row, p->d_col, p->d_depth, p->d_pred};
     for (int *b : bufs)
         if (b) (void) hipFree(b);
     if (p->d_visited) (void) hipFree(p->d_visited);
     if (p->d_ctl) (void) hipFree(p->d_ctl);
     if (p->d_hubs) (void) hipFree(p->d_hubs);
+    if (p->d_fbm) (void) hipFree(p->d_fbm);
     if (p->h_ctl) (void) hipHostFree(p->h_ctl);
     if (p->ev_start) (void) hipEventDestroy(p->ev_start);
     if (p->ev_stop) (void) hipEventDestroy(p->ev_stop);

@@ -48,6 +48,10 @@ struct alignas(256) Control {
     unsigned long long hub_alloc[2];  // grid tier: (entries << 32) | chunks, by step & 1
     unsigned hub_steal[2];            // grid tier: chunk counter, by step & 1
     unsigned long long pad1[2];
+    // Direction-optimizing bookkeeping (second 128-byte line).
+    unsigned long long fedges[3];     // sum of degrees of each frontier, by step % 3
+    unsigned long long visited_edges; // sum of degrees of all vertices labelled so far
+    unsigned long long pad2[12];
     unsigned bar_root;        // own 128-byte line
     unsigned pad3[31];
     unsigned bar_leaf[kBarrierFanIn * 32];     // one 128-byte line per leaf counter

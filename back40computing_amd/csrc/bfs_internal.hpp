@@ -17,13 +17,16 @@ enum Phase : int {
     kPhaseContractExpand = 1,  // edge frontier -> edge frontier (claim + gather)
     kPhaseContract = 2,        // edge frontier -> vertex frontier (claim)
     kPhaseExpand = 3,          // vertex frontier -> edge frontier (gather)
-    kPhaseFilter = 4           // edge frontier -> edge frontier (visited-bitmask cull)
+    kPhaseFilter = 4,          // edge frontier -> edge frontier (visited-bitmask cull)
+    kPhaseBottomUp = 5         // frontier bitmap -> every unvisited vertex looks for a parent
 };
 
 struct GraphArgs {
     const int *row;   // [V + 1]
     const int *col;   // [E]
     int nodes;
+    const int *rrow;  // reverse (in-neighbour) CSR for bottom-up levels;
+    const int *rcol;  // equal to row / col when the graph is symmetric
 };
 
 struct SearchArgs {
@@ -36,6 +39,11 @@ struct SearchArgs {
     Control *ctl;
     HubEntry *hubs[2];     // grid-tier lists, by step & 1
     unsigned hub_cap;      // entries per list
+    unsigned *fbm[2];      // frontier bitmaps of the bottom-up levels, by iteration & 1
+    int nodes;
+    unsigned long long total_edges;
+    int track_edges;       // accumulate frontier edge counts (direction switching)
+    int do_alpha, do_beta; // top-down -> bottom-up when m_f * alpha > m_u; back when n_f * beta < V
     int wave_t, cta_t, grid_t, fast_t, sat_t;
     int instrument;
 };
@@ -49,10 +57,12 @@ struct Cursor {
 };
 
 // Strategy of a persistent launch.
-enum LoopKind : int { kLoopEC = 0, kLoopCE = 1, kLoopTP = 2 };
+enum LoopKind : int { kLoopEC = 0, kLoopCE = 1, kLoopTP = 2, kLoopDO = 3 };
 
 // Host launchers (bfs_kernels.hip).
-hipError_t launch_init_search(const SearchArgs &s, int src, int nodes, int label_source, hipStream_t st);
+hipError_t launch_init_search(const SearchArgs &s, const int *row, int src, int label_source, hipStream_t st);
+// Sets the frontier bitmap fbm[iteration & 1] from the queue of step c.
+hipError_t launch_build_frontier_bitmap(const SearchArgs &s, Cursor c, int grid, hipStream_t st, int *launches);
 hipError_t launch_reset_labels(int *depth, int *pred, unsigned *visited, int nodes, hipStream_t st);
 // One step as a kernel launch; gathering phases are followed by the
 // grid-tier launch for rows published to the hub list.

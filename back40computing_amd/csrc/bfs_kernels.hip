@@ -90,6 +90,26 @@ __device__ __forceinline__ void note_depth(Control *ctl, bool labeled, int d)
     if (ballot(labeled) != 0 && lane_id() == 0) atomicMax(&ctl->max_depth, d);
 }
 
+// Adds the wave's per-lane degree sums of newly labelled vertices to the
+// next frontier's edge count and to the visited-edge total (64-bit, from two
+// 16-bit halves so the DPP sums cannot overflow).
+__device__ __forceinline__ void note_edges(Control *ctl, int step, unsigned fe)
+{
+    int lo = wave_inclusive_sum((int) (fe & 0xFFFFu));
+    int hi = wave_inclusive_sum((int) (fe >> 16));
+    unsigned long long tot = ((unsigned long long) (unsigned) readlane(hi, kWave - 1) << 16) +
+                             (unsigned long long) (unsigned) readlane(lo, kWave - 1);
+    if (tot != 0 && lane_id() == 0) {
+        __hip_atomic_fetch_add(&ctl->fedges[(step + 1) % 3], tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&ctl->visited_edges, tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__device__ __forceinline__ unsigned long long ld64_agent(const unsigned long long *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ---------------------------------------------------------------------------
 // Generic phases
 // ---------------------------------------------------------------------------
@@ -166,12 +186,14 @@ __device__ void body_expand_contract(const GraphArgs &g, const SearchArgs &s, Cu
     const int *in = s.q[c.step & 1];
     const int next_depth = c.iteration + 1;
     bool labeled = false;
+    unsigned fe = 0;
     auto edge = [&](bool ok, int u, int par) {
         bool fresh = ok && claim<EXACT>(s.visited, s.depth, u);
         if (fresh) {
             s.depth[u] = next_depth;
             if (PRED) s.pred[u] = par;
             labeled = true;
+            if (s.track_edges) fe += (unsigned) (g.row[u + 1] - g.row[u]);
         }
         wq.push(fresh, u, par);
     };
@@ -187,6 +209,96 @@ __device__ void body_expand_contract(const GraphArgs &g, const SearchArgs &s, Cu
     }
     wq.flush();
     note_depth(s.ctl, labeled, next_depth);
+    if (s.track_edges) note_edges(s.ctl, c.step, fe);
+}
+
+// Bottom-up step of a direction-optimizing search (Beamer, Asanovic,
+// Patterson, "Direction-Optimizing Breadth-First Search", SC'12): every
+// unvisited vertex scans its row for a neighbour in the current frontier
+// bitmap fbm[d & 1] and stops at the first one.  Each wave owns 64
+// consecutive vertices, i.e. two whole bitmap words, so the visited and
+// next-frontier words are written with plain stores (the next-frontier word
+// is overwritten in full, which also clears it).  Short rows are scanned by
+// their own lane, four neighbours per round; rows of at least kBuLong
+// neighbours by the whole wave, 64 per round, with a ballot early exit.
+constexpr int kBuLong = 64;
+
+template <bool PRED, bool EXACT>
+__device__ void body_bottom_up(const GraphArgs &g, const SearchArgs &s, Cursor c, Lds<PRED> &lds)
+{
+    WaveQueue<PRED> wq = make_queue<PRED>(s, c.step, lds);
+    const unsigned *cur = s.fbm[c.iteration & 1];
+    unsigned *nxt = s.fbm[(c.iteration + 1) & 1];
+    const int lane = lane_id();
+    const int next_depth = c.iteration + 1;
+    auto in_frontier = [&](int u) { return ((cur[u >> 5] >> (u & 31)) & 1u) != 0; };
+    bool labeled = false;
+    unsigned fe = 0;
+    while (true) {
+        unsigned base = next_item(&s.ctl->steal[c.step & 1], lds) * kBlock;
+        if (base >= (unsigned) g.nodes) break;
+        int v = (int) (base + threadIdx.x);
+        unsigned w = (unsigned) v >> 5;
+        bool valid = v < g.nodes;
+        unsigned vis = valid ? s.visited[w] : ~0u;
+        bool todo = valid && ((vis >> (v & 31)) & 1u) == 0;
+        // Best-effort top-down levels may have lost visited bits to racing
+        // read-modify-writes; the label stays the authority there.
+        if (!EXACT && todo) todo = s.depth[v] == -1;
+        int beg = 0, end = 0;
+        if (todo) {
+            beg = g.rrow[v];
+            end = g.rrow[v + 1];
+        }
+        int len = end - beg;
+        int parent = -1;
+        if (todo && len < kBuLong) {
+            for (int e = beg; e < end; e += 4) {
+                int u0 = g.rcol[e];
+                int u1 = e + 1 < end ? g.rcol[e + 1] : -1;
+                int u2 = e + 2 < end ? g.rcol[e + 2] : -1;
+                int u3 = e + 3 < end ? g.rcol[e + 3] : -1;
+                if (in_frontier(u0)) { parent = u0; break; }
+                if (u1 >= 0 && in_frontier(u1)) { parent = u1; break; }
+                if (u2 >= 0 && in_frontier(u2)) { parent = u2; break; }
+                if (u3 >= 0 && in_frontier(u3)) { parent = u3; break; }
+            }
+        }
+        uint64_t lm = ballot(todo && len >= kBuLong);
+        while (lm) {
+            int l = uniform(__builtin_ctzll(lm));
+            lm &= lm - 1;
+            int b = readlane(beg, l), e = readlane(end, l);
+            int found = -1;
+            for (int k = b; k < e; k += kWave) {
+                int idx = k + lane;
+                int u = idx < e ? g.rcol[idx] : -1;
+                uint64_t hm = ballot(u >= 0 && in_frontier(u));
+                if (hm) {
+                    found = readlane(u, uniform(__builtin_ctzll(hm)));
+                    break;
+                }
+            }
+            if (lane == l) parent = found;
+        }
+        bool got = parent >= 0;
+        if (got) {
+            s.depth[v] = next_depth;
+            if (PRED) s.pred[v] = parent;
+            fe += (unsigned) len;
+            labeled = true;
+        }
+        uint64_t gm = ballot(got);
+        if ((lane & 31) == 0 && valid) {
+            unsigned bits = (unsigned) (lane == 0 ? gm : gm >> 32);
+            if (bits) s.visited[w] = vis | bits;
+            nxt[w] = bits;
+        }
+        wq.push(got, v, parent);
+    }
+    wq.flush();
+    note_depth(s.ctl, labeled, next_depth);
+    note_edges(s.ctl, c.step, fe);
 }
 
 template <bool PRED, bool EXACT>
@@ -310,8 +422,52 @@ __device__ __forceinline__ void run_body(int phase, const GraphArgs &g, const Se
     case kPhaseContractExpand: body_contract_expand<PRED, EXACT>(g, s, c, n, lds, sub, hubs); break;
     case kPhaseContract: body_contract<PRED, EXACT>(g, s, c, n, lds); break;
     case kPhaseExpand: body_expand<PRED, EXACT>(g, s, c, n, lds, sub, hubs); break;
+    case kPhaseBottomUp: body_bottom_up<PRED, EXACT>(g, s, c, lds); break;
     default: body_filter<PRED, EXACT>(g, s, c, n, lds); break;
     }
+}
+
+// Two grid-wide passes that turn the vertex queue of step c into the
+// frontier bitmap fbm[c.iteration & 1] (zero, then set); a grid barrier or
+// kernel boundary must separate them.
+__device__ __forceinline__ void frontier_bitmap_pass(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n,
+                                                     int pass)
+{
+    unsigned *bm = s.fbm[c.iteration & 1];
+    const unsigned stride = gridDim.x * kBlock;
+    if (pass == 0) {
+        const unsigned words = ((unsigned) g.nodes + 31) >> 5;
+        for (unsigned w = blockIdx.x * kBlock + threadIdx.x; w < words; w += stride) bm[w] = 0u;
+    } else {
+        const int *in = s.q[c.step & 1];
+        for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+            int u = in[i];
+            __hip_atomic_fetch_or(&bm[u >> 5], 1u << (u & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// Direction choice of a direction-optimizing search at the top of a level
+// (grid-uniform: every input is read after the level barrier).
+__device__ __forceinline__ int choose_direction(const SearchArgs &s, Cursor c, unsigned n)
+{
+    unsigned long long mf = ld64_agent(&s.ctl->fedges[c.step % 3]);
+    unsigned long long ve = ld64_agent(&s.ctl->visited_edges);
+    unsigned long long mu = s.total_edges > ve ? s.total_edges - ve : 0ull;
+    if (c.phase == kPhaseExpandContract) {
+        if (mf * (unsigned long long) s.do_alpha > mu) return kPhaseBottomUp;
+    } else if ((unsigned long long) n * (unsigned long long) s.do_beta < (unsigned long long) s.nodes) {
+        return kPhaseExpandContract;
+    }
+    return c.phase;
+}
+
+// Whether a level is small enough for the single-workgroup fast path.
+__device__ __forceinline__ bool fast_level(const SearchArgs &s, Cursor c, unsigned n)
+{
+    if (s.fast_t < 0 || n > (unsigned) s.fast_t || c.phase == kPhaseBottomUp) return false;
+    if (s.track_edges && ld64_agent(&s.ctl->fedges[c.step % 3]) > 32ull * (unsigned) s.fast_t) return false;
+    return true;
 }
 
 __device__ __forceinline__ Cursor advance(int loop, Cursor c)
@@ -335,6 +491,7 @@ __device__ __forceinline__ void prepare_step(Control *ctl, Cursor c, unsigned n)
     st_agent(&ctl->steal[(c.step + 1) & 1], 0u);
     st_agent(&ctl->hub_steal[(c.step + 1) & 1], 0u);
     __hip_atomic_store(&ctl->hub_alloc[(c.step + 1) & 1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&ctl->fedges[(c.step + 2) % 3], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     ctl->total_queued += n;
     if (c.step < kLevelLog) {
         ctl->level_len[c.step] = n;
@@ -372,8 +529,18 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
             reason = kExitSaturated;
             break;
         }
+        if (loop == kLoopDO) {
+            int dir = choose_direction(s, c, n);
+            if (dir == kPhaseBottomUp && c.phase != kPhaseBottomUp) {
+                frontier_bitmap_pass(g, s, c, n, 0);
+                grid_barrier(s.ctl);
+                frontier_bitmap_pass(g, s, c, n, 1);
+                grid_barrier(s.ctl);
+            }
+            c.phase = dir;
+        }
 
-        if (s.fast_t >= 0 && n <= (unsigned) s.fast_t && gridDim.x > 1) {
+        if (fast_level(s, c, n) && gridDim.x > 1) {
             // Single-workgroup fast path: block 0 runs consecutive small
             // levels with workgroup barriers only (no grid tier: it alone
             // works); the rest of the grid waits at the next grid barrier.
@@ -387,8 +554,9 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
                     if (threadIdx.x == 0) lds.nb = frontier_length(s, c.step);
                     __syncthreads();
                     n = lds.nb;
-                    if (n == 0 || n > (unsigned) s.fast_t || ld_agent(&s.ctl->overflow)) break;
+                    if (n == 0 || !fast_level(s, c, n) || ld_agent(&s.ctl->overflow)) break;
                     if (loop == kLoopCE && s.sat_t >= 0 && n > (unsigned) s.sat_t) break;
+                    if (loop == kLoopDO && choose_direction(s, c, n) != c.phase) break;
                 }
                 if (threadIdx.x == 0) {
                     st_agent(&s.ctl->fast_step, c.step);
@@ -456,6 +624,11 @@ __global__ __launch_bounds__(kBlock) void hub_kernel(GraphArgs g, SearchArgs s, 
     run_body<PRED, EXACT>(PHASE, g, s, c, 0, lds, 1, true);
 }
 
+__global__ __launch_bounds__(kBlock) void frontier_bitmap_kernel(GraphArgs g, SearchArgs s, Cursor c, int pass)
+{
+    frontier_bitmap_pass(g, s, c, frontier_length(s, c.step), pass);
+}
+
 // ---------------------------------------------------------------------------
 // Initialisation kernels.
 // ---------------------------------------------------------------------------
@@ -474,7 +647,7 @@ __global__ void reset_labels_kernel(int *depth, int *pred, unsigned *visited, in
 // every counter here, before the first step, replaces the reference's
 // iteration-0 counter reset inside every CTA (contract_expand_atomic/
 // kernel.cuh:307-332), which can race with CTA 0's first enqueue.
-__global__ void init_search_kernel(SearchArgs s, int src, int label_source)
+__global__ void init_search_kernel(SearchArgs s, const int *row, int src, int label_source)
 {
     Control *ctl = s.ctl;
     unsigned *words = reinterpret_cast<unsigned *>(ctl);
@@ -490,6 +663,9 @@ __global__ void init_search_kernel(SearchArgs s, int src, int label_source)
             if (s.pred) s.pred[src] = -2;
             s.visited[src >> 5] |= 1u << (src & 31);
             ctl->max_depth = 0;
+            unsigned long long deg = (unsigned long long) (row[src + 1] - row[src]);
+            ctl->fedges[0] = deg;
+            ctl->visited_edges = deg;
         }
     }
 }
@@ -566,6 +742,7 @@ hipError_t launch_step_t(int phase, const GraphArgs &g, const SearchArgs &s, Cur
     case kPhaseContract: return launch_phase<PRED, EXACT, kPhaseContract>(g, s, c, grid, st, launches);
     case kPhaseExpand: return launch_phase<PRED, EXACT, kPhaseExpand>(g, s, c, grid, st, launches);
     case kPhaseFilter: return launch_phase<PRED, EXACT, kPhaseFilter>(g, s, c, grid, st, launches);
+    case kPhaseBottomUp: return launch_phase<PRED, EXACT, kPhaseBottomUp>(g, s, c, grid, st, launches);
     default: return hipErrorInvalidValue;
     }
 }
@@ -581,10 +758,19 @@ hipError_t launch_reset_labels(int *depth, int *pred, unsigned *visited, int nod
     return hipGetLastError();
 }
 
-hipError_t launch_init_search(const SearchArgs &s, int src, int nodes, int label_source, hipStream_t st)
+hipError_t launch_init_search(const SearchArgs &s, const int *row, int src, int label_source, hipStream_t st)
 {
-    (void) nodes;
-    hipLaunchKernelGGL(init_search_kernel, dim3(1), dim3(256), 0, st, s, src, label_source);
+    hipLaunchKernelGGL(init_search_kernel, dim3(1), dim3(256), 0, st, s, row, src, label_source);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_frontier_bitmap(const SearchArgs &s, Cursor c, int grid, hipStream_t st, int *launches)
+{
+    GraphArgs g{nullptr, nullptr, s.nodes, nullptr, nullptr};
+    for (int pass = 0; pass < 2; pass++) {
+        hipLaunchKernelGGL(frontier_bitmap_kernel, dim3(grid), dim3(kBlock), 0, st, g, s, c, pass);
+        (*launches)++;
+    }
     return hipGetLastError();
 }
 

@@ -30,7 +30,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "BFS GTEPS (R-MAT s22 ef16) + batched queries/s & p99 vs single-source"
 HBM_PEAK_GBS = 8000.0
-STRATEGIES = {"ec": 0, "ce": 1, "tp": 2, "hybrid": 3}
+STRATEGIES = {"ec": 0, "ce": 1, "tp": 2, "hybrid": 3, "do": 4}
 
 
 def log(*a):
@@ -92,7 +92,7 @@ def main():
     srcs = g.sample_sources(args.sources)
     strategy = STRATEGIES[args.strategy]
     fused = not args.iterative
-    ft = b40c.VERTEX_FRONTIERS if strategy == b40c.EXPAND_CONTRACT else b40c.EDGE_FRONTIERS
+    ft = b40c.bfs._native.load().bfs_strategy_frontier_type(strategy)
 
     p = b40c.CsrProblem().from_host_problem(g)
     overrides = {}

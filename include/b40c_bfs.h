@@ -65,7 +65,11 @@ typedef enum bfs_strategy {
     BFS_EXPAND_CONTRACT = 0,
     BFS_CONTRACT_EXPAND = 1,
     BFS_TWO_PHASE = 2,
-    BFS_HYBRID = 3
+    BFS_HYBRID = 3,
+    /* addition: expand-contract top-down levels mixed with bottom-up levels,
+       switching direction by frontier edge and vertex counts (Beamer et al.,
+       SC'12); switch points are the do_alpha / do_beta tunables */
+    BFS_DIRECTION_OPTIMIZING = 4
 } bfs_strategy;
 
 /* Same numbering as FrontierType (b40c/graph/bfs/csr_problem.cuh:54-58). */
@@ -98,7 +102,11 @@ typedef struct bfs_config {
     int32_t instrument;             /* 1: collect per-workgroup duty statistics */
     int32_t grid_gather_threshold;  /* row length >= this: the row is split into
                                        2048-edge chunks shared by the whole grid */
-    int32_t reserved[8];
+    int32_t do_alpha;               /* direction-optimizing: go bottom-up when
+                                       frontier edges * alpha > unexplored edges */
+    int32_t do_beta;                /* direction-optimizing: back to top-down when
+                                       frontier vertices * beta < V */
+    int32_t reserved[6];
 } bfs_config;
 
 typedef struct bfs_stats {

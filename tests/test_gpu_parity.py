@@ -24,12 +24,15 @@ VARIANTS = [
     (b40c.CONTRACT_EXPAND, True), (b40c.CONTRACT_EXPAND, False),
     (b40c.TWO_PHASE, True), (b40c.TWO_PHASE, False),
     (b40c.HYBRID, True),
+    (b40c.DIRECTION_OPTIMIZING, True), (b40c.DIRECTION_OPTIMIZING, False),
 ]
-VARIANT_IDS = ["ec_fused", "ec_iter", "ce_fused", "ce_iter", "tp_fused", "tp_iter", "hybrid"]
+VARIANT_IDS = ["ec_fused", "ec_iter", "ce_fused", "ce_iter", "tp_fused", "tp_iter", "hybrid", "do_fused", "do_iter"]
 
 
 def frontier_type(strategy):
-    return b40c.VERTEX_FRONTIERS if strategy == b40c.EXPAND_CONTRACT else b40c.EDGE_FRONTIERS
+    if strategy in (b40c.EXPAND_CONTRACT, b40c.DIRECTION_OPTIMIZING):
+        return b40c.VERTEX_FRONTIERS
+    return b40c.EDGE_FRONTIERS
 
 
 def run(problem, strategy, fused, src, queue_sizing=1.3, max_grid_size=0):
@@ -139,6 +142,23 @@ def test_hybrid_switches_back_and_forth(gpu):
     p.destroy()
 
 
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "iter"])
+def test_direction_switch_points(gpu, fused):
+    """alpha = 10^6 with beta = 1 flips direction at every level (each
+    top-down level is followed by a bottom-up one), stressing every
+    hand-over between the two directions; alpha = 1 stays top-down."""
+    graphs = [CsrGraph.rmat(15, 16, seed=31), CsrGraph.grid2d(40), CsrGraph.grid3d(9),
+              CsrGraph.erdos_renyi(5000, 20000, seed=2)]
+    for g in graphs:
+        p = CsrProblem(mark_predecessors=True).from_host_problem(g)
+        srcs = [0, g.nodes // 3] if g.degrees().min() > 0 else list(g.sample_sources(2, seed=7))
+        for alpha, beta in [(1000000, 1), (14, 24), (1, 24), (1000000, 1000000)]:
+            p.set_config(do_alpha=alpha, do_beta=beta)
+            for src in srcs:
+                assert_parity(g, p, b40c.DIRECTION_OPTIMIZING, fused, src)
+        p.destroy()
+
+
 def test_grid_lattices_small(gpu):
     for w, builder in [(37, CsrGraph.grid2d), (11, CsrGraph.grid3d)]:
         g = builder(w)
@@ -210,7 +230,8 @@ def test_config2_rmat22(gpu):
     p = CsrProblem().from_host_problem(g)
     srcs = g.sample_sources(64)
     for i, (strategy, fused) in enumerate([(b40c.EXPAND_CONTRACT, True), (b40c.HYBRID, True),
-                                           (b40c.TWO_PHASE, False), (b40c.CONTRACT_EXPAND, True)]):
+                                           (b40c.TWO_PHASE, False), (b40c.CONTRACT_EXPAND, True),
+                                           (b40c.DIRECTION_OPTIMIZING, True), (b40c.DIRECTION_OPTIMIZING, False)]):
         assert_parity(g, p, strategy, fused, srcs[i])
     p.destroy()
 
