@@ -50,6 +50,54 @@ class BfsStats(ctypes.Structure):
     ]
 
 
+class BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("elapsed_ms", ctypes.c_double),
+        ("queries", ctypes.c_int32),
+        ("levels", ctypes.c_int32),
+        ("launches", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32),
+        ("latency_p50_ms", ctypes.c_double),
+        ("latency_p99_ms", ctypes.c_double),
+        ("latency_max_ms", ctypes.c_double),
+    ]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("max_slots", ctypes.c_int32),
+        ("levels_per_tick", ctypes.c_int32),
+        ("hbm_budget_bytes", ctypes.c_int64),
+        ("direction_optimizing", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
+    ]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [
+        ("ticket", ctypes.c_int64),
+        ("graph_id", ctypes.c_int32),
+        ("src", ctypes.c_int32),
+        ("levels", ctypes.c_int32),
+        ("status", ctypes.c_int32),
+        ("submit_ms", ctypes.c_double),
+        ("admit_ms", ctypes.c_double),
+        ("finish_ms", ctypes.c_double),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_int64),
+        ("admitted", ctypes.c_int64),
+        ("completed", ctypes.c_int64),
+        ("rejected", ctypes.c_int64),
+        ("ticks", ctypes.c_int64),
+        ("max_resident", ctypes.c_int64),
+        ("busy_ms", ctypes.c_double),
+    ]
+
+
 _lib = None
 
 
@@ -105,6 +153,24 @@ def load():
          ctypes.POINTER(dbl), c_int32_p)
     _sig(lib, "bfs_barrier_benchmark", ctypes.c_int, i32, i32, i32, ctypes.POINTER(dbl), c_int32_p)
     _sig(lib, "bfs_copy_calibration", ctypes.c_int, i32, i64, ctypes.POINTER(dbl))
+    vpp = ctypes.POINTER(vp)
+    _sig(lib, "bfs_pool_create", ctypes.c_int, i32, vpp)
+    _sig(lib, "bfs_pool_add_graph", ctypes.c_int, vp, c_int32_p, c_int32_p, i32, i64, c_int32_p)
+    _sig(lib, "bfs_pool_graph_info", ctypes.c_int, vp, i32, c_int32_p, ctypes.POINTER(i64))
+    _sig(lib, "bfs_pool_device_bytes", ctypes.c_int, vp, ctypes.POINTER(i64))
+    _sig(lib, "bfs_pool_destroy", None, vp)
+    _sig(lib, "bfs_query_state_bytes", i64, i32)
+    _sig(lib, "bfs_batch_create", ctypes.c_int, vp, i32, vpp)
+    _sig(lib, "bfs_batch_set_config", ctypes.c_int, vp, ctypes.POINTER(BfsConfig))
+    _sig(lib, "bfs_batch_enact", ctypes.c_int, vp, c_int32_p, c_int32_p, i32, i32, i32, ctypes.POINTER(BatchStats))
+    _sig(lib, "bfs_batch_query_latency", ctypes.c_int, vp, ctypes.POINTER(dbl))
+    _sig(lib, "bfs_batch_extract_depths", ctypes.c_int, vp, i32, c_int32_p)
+    _sig(lib, "bfs_batch_destroy", None, vp)
+    _sig(lib, "bfs_server_create", ctypes.c_int, vp, ctypes.POINTER(ServerConfig), vpp)
+    _sig(lib, "bfs_server_submit", ctypes.c_int, vp, i32, i32, c_int32_p, ctypes.POINTER(i64))
+    _sig(lib, "bfs_server_wait", ctypes.c_int, vp, i64, i32, ctypes.POINTER(RequestInfo))
+    _sig(lib, "bfs_server_get_stats", ctypes.c_int, vp, ctypes.POINTER(ServerStats))
+    _sig(lib, "bfs_server_destroy", None, vp)
     _sig(lib, "bfs_selftest_wave_primitives", ctypes.c_int, i32, c_int32_p)
     _lib = lib
     return lib

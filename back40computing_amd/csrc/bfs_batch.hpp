@@ -1,0 +1,56 @@
+// bfs_batch.hpp -- device-side types of batched BFS (many single-source
+// queries over a pool of resident graphs, advanced level by level together).
+// Shared by bfs_batch.hip (kernels and the batch / server host code).
+#pragma once
+
+#include "bfs_device.hpp"
+
+namespace b40c_hip {
+
+constexpr int kMaxSlots = 256;  // queries advanced by one launch
+
+struct BatchGraph {
+    const int *row, *col;    // CSR
+    const int *rrow, *rcol;  // in-neighbour CSR (aliases when symmetric)
+    int nodes;
+    int pad;
+    unsigned long long edges;
+};
+
+// One query's state.  Counters rotate by the query's own step exactly as in
+// the single-source Control block.
+struct alignas(128) BatchSlot {
+    int graph, src, active, done;
+    int step, iteration, phase, overflow;
+    int next_phase, ran;      // direction chosen for the level in flight, and whether it had work
+    int *depth;
+    unsigned *visited;
+    int *q[2];
+    unsigned cap;             // queue capacity (elements)
+    unsigned qlen[3];
+    unsigned long long fedges[3];
+    unsigned long long visited_edges;
+    unsigned long long t_begin, t_end;  // wall clock at seeding / completion
+};
+
+struct alignas(256) BatchControl {
+    unsigned steal[2];        // tile counters by level parity
+    int level;                // levels completed by the batch
+    int pad0;
+    unsigned long long t_start;
+    unsigned bar_pad[26];
+    // grid barrier state (same layout discipline as Control)
+    Control bar;
+};
+
+struct BatchArgs {
+    const BatchGraph *graphs;
+    BatchSlot *slots;
+    BatchControl *ctl;
+    int nslots;
+    int wave_t, cta_t;
+    int direction;            // 0: top-down only, 1: direction-optimizing
+    int do_alpha, do_beta;
+};
+
+}  // namespace b40c_hip

@@ -803,3 +803,22 @@ bfs_status bfs_selftest_wave_primitives(int32_t device, int32_t *mismatches)
 }
 
 }  // extern "C"
+
+// Helpers shared with the batch / server translation unit.
+namespace b40c_hip {
+
+bfs_status host_fail(bfs_status s, const char *msg) { return fail(s, msg); }
+
+bfs_status host_hip_fail(hipError_t e, const char *what) { return hip_fail(e, what); }
+
+bfs_status host_validate_csr(const int32_t *row, const int32_t *col, int32_t nodes, int64_t edges)
+{
+    return validate_csr(row, col, nodes, edges);
+}
+
+bool host_csr_is_symmetric(const int32_t *row, const int32_t *col, int32_t nodes)
+{
+    return csr_is_symmetric(row, col, nodes);
+}
+
+}  // namespace b40c_hip
