@@ -35,7 +35,10 @@ class BfsConfig(ctypes.Structure):
         ("grid_gather_threshold", ctypes.c_int32),
         ("do_alpha", ctypes.c_int32),
         ("do_beta", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("tile_grab_max", ctypes.c_int32),
+        ("bu_tile_blocks", ctypes.c_int32),
+        ("grab_strided", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -166,6 +169,8 @@ def load():
     _sig(lib, "bfs_batch_query_latency", ctypes.c_int, vp, ctypes.POINTER(dbl))
     _sig(lib, "bfs_batch_extract_depths", ctypes.c_int, vp, i32, c_int32_p)
     _sig(lib, "bfs_batch_destroy", None, vp)
+    _sig(lib, "bfs_batch_level_log", ctypes.c_int, vp, i32, ctypes.POINTER(ctypes.c_uint32),
+         ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(dbl), c_int32_p)
     _sig(lib, "bfs_server_create", ctypes.c_int, vp, ctypes.POINTER(ServerConfig), vpp)
     _sig(lib, "bfs_server_submit", ctypes.c_int, vp, i32, i32, c_int32_p, ctypes.POINTER(i64))
     _sig(lib, "bfs_server_wait", ctypes.c_int, vp, i64, i32, ctypes.POINTER(RequestInfo))

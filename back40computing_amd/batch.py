@@ -76,6 +76,15 @@ class Batch:
         _check(_native.load().bfs_batch_query_latency(self._h, out), "bfs_batch_query_latency")
         return np.array(out[:], dtype=np.float64)
 
+    def level_log(self):
+        """[(top-down tiles, bottom-up tiles, start microseconds)] of the last enact."""
+        td = (ctypes.c_uint32 * 64)()
+        bu = (ctypes.c_uint32 * 64)()
+        us = (ctypes.c_double * 64)()
+        k = ctypes.c_int32()
+        _check(_native.load().bfs_batch_level_log(self._h, 64, td, bu, us, ctypes.byref(k)), "bfs_batch_level_log")
+        return [(int(td[i]), int(bu[i]), float(us[i])) for i in range(k.value)]
+
     def depths(self, q):
         out = np.empty(self.pool.nodes[int(self._graphs[q])], dtype=np.int32)
         _check(_native.load().bfs_batch_extract_depths(self._h, q, _native.i32ptr(out)), "bfs_batch_extract_depths")

@@ -46,9 +46,11 @@ struct PoolGraph {
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+constexpr unsigned kBatchHubCapacity = 1u << 16;  // grid-tier rows per level
+
 // Per-query device state layout inside an arena (256-byte aligned pieces).
 struct SlotLayout {
-    size_t depth, visited, q0, q1, bytes;
+    size_t depth, visited, fbm0, fbm1, q0, q1, bytes;
 };
 
 SlotLayout slot_layout(int32_t nodes)
@@ -57,7 +59,10 @@ SlotLayout slot_layout(int32_t nodes)
     size_t v = (size_t) std::max(nodes, 1);
     l.depth = 0;
     l.visited = align_up(4 * v, 256);
-    l.q0 = l.visited + align_up(4 * ((v + 31) / 32), 256);
+    size_t words = align_up(4 * ((v + 31) / 32), 256);
+    l.fbm0 = l.visited + words;
+    l.fbm1 = l.fbm0 + words;
+    l.q0 = l.fbm1 + words;
     l.q1 = l.q0 + align_up(4 * v, 256);
     l.bytes = l.q1 + align_up(4 * v, 256);
     return l;
@@ -101,13 +106,12 @@ struct SlotTable {
     int *d_list = nullptr;          // slot indices to seed
     int *d_alive = nullptr;
     int *h_alive = nullptr;         // pinned
+    HubEntry *d_hubs = nullptr;     // two grid-tier lists of kBatchHubCapacity entries
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_levels = 0;
-    int graph_nslots = -1;
     int graph_grid = 0;
-    BatchGraph *graph_table = nullptr;
-    int graph_direction = -1;
+    BatchArgs graph_args{};         // arguments the captured graph was built with
     bfs_config cfg{};
 
     bfs_status init(int dev, int slots)
@@ -122,6 +126,7 @@ struct SlotTable {
         HIP_OK(hipMalloc(&d_list, sizeof(int) * slots), "allocate seed list");
         HIP_OK(hipMalloc(&d_alive, sizeof(int)), "allocate alive counter");
         HIP_OK(hipHostMalloc(&h_alive, sizeof(int), hipHostMallocDefault), "allocate pinned counter");
+        HIP_OK(hipMalloc(&d_hubs, sizeof(HubEntry) * 2 * kBatchHubCapacity), "allocate grid-tier lists");
         memset(h_slots, 0, sizeof(BatchSlot) * slots);
         HIP_OK(hipMemcpy(d_slots, h_slots, sizeof(BatchSlot) * slots, hipMemcpyHostToDevice), "clear slots");
         HIP_OK(hipMemset(d_ctl, 0, sizeof(BatchControl)), "clear batch control");
@@ -141,6 +146,7 @@ struct SlotTable {
         if (d_list) (void) hipFree(d_list);
         if (d_alive) (void) hipFree(d_alive);
         if (h_alive) (void) hipHostFree(h_alive);
+        if (d_hubs) (void) hipFree(d_hubs);
         if (stream) (void) hipStreamDestroy(stream);
         graph_exec = nullptr;
         graph = nullptr;
@@ -150,6 +156,7 @@ struct SlotTable {
     BatchArgs args(const BatchGraph *table, int nslots, int direction) const
     {
         BatchArgs a;
+        std::memset(&a, 0, sizeof a);  // padding too: cached graphs compare argument bytes
         a.graphs = table;
         a.slots = d_slots;
         a.ctl = d_ctl;
@@ -159,14 +166,22 @@ struct SlotTable {
         a.direction = direction;
         a.do_alpha = cfg.do_alpha;
         a.do_beta = cfg.do_beta;
+        a.hubs[0] = d_hubs;
+        a.hubs[1] = d_hubs + kBatchHubCapacity;
+        a.hub_cap = kBatchHubCapacity;
+        a.grid_t = cfg.grid_gather_threshold > 0 ? cfg.grid_gather_threshold : 0x7fffffff;
+        a.grab_max = cfg.tile_grab_max > 0 ? cfg.tile_grab_max : 1;
+        a.bu_tile = cfg.bu_tile_blocks > 0 ? cfg.bu_tile_blocks : 1;
+        a.strided = cfg.grab_strided != 2;
         return a;
     }
 
     // Captures `levels` (level kernel, advance kernel) pairs into a graph.
     bfs_status capture(const BatchArgs &a, int grid, int levels)
     {
-        if (graph_exec && graph_nslots == a.nslots && graph_levels == levels && graph_grid == grid &&
-            graph_table == a.graphs && graph_direction == a.direction)
+        // the kernel arguments are baked into the graph: any change re-captures
+        if (graph_exec && graph_levels == levels && graph_grid == grid &&
+            std::memcmp(&graph_args, &a, sizeof a) == 0)
             return BFS_OK;
         if (graph_exec) (void) hipGraphExecDestroy(graph_exec);
         if (graph) (void) hipGraphDestroy(graph);
@@ -184,11 +199,9 @@ struct SlotTable {
         }
         HIP_OK(hipStreamEndCapture(stream, &graph), "end capture");
         HIP_OK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0), "instantiate level graph");
-        graph_nslots = a.nslots;
         graph_levels = levels;
         graph_grid = grid;
-        graph_table = const_cast<BatchGraph *>(a.graphs);
-        graph_direction = a.direction;
+        graph_args = a;
         return BFS_OK;
     }
 };
@@ -202,6 +215,7 @@ struct bfs_batch {
     int nq = 0;
     std::vector<double> latency_ms;
     std::vector<int32_t> nodes_of;
+    BatchControl last{};   // control block of the last enact (level record)
 };
 
 namespace {
@@ -234,7 +248,9 @@ int batch_grid(const bfs_graph_pool *p, bool persistent, int blocks_per_cu)
         int cap = std::max(1, std::min(4, occ / 2));
         return p->num_cus * std::min(cap, std::max(1, blocks_per_cu));
     }
-    return p->num_cus * std::max(1, occ);
+    // level launches drain a shared tile counter, so the grid only sets how
+    // many queries' working sets are in flight at once
+    return p->num_cus * std::max(1, std::min(occ, std::max(1, blocks_per_cu)));
 }
 
 }  // namespace
@@ -364,6 +380,10 @@ bfs_status bfs_batch_set_config(bfs_batch *b, const bfs_config *cfg)
     if (c.do_alpha <= 0) c.do_alpha = d.do_alpha;
     if (c.do_beta <= 0) c.do_beta = d.do_beta;
     if (c.blocks_per_cu <= 0) c.blocks_per_cu = d.blocks_per_cu;
+    if (c.grid_gather_threshold == 0) c.grid_gather_threshold = d.grid_gather_threshold;
+    if (c.tile_grab_max <= 0) c.tile_grab_max = d.tile_grab_max;
+    if (c.bu_tile_blocks <= 0) c.bu_tile_blocks = d.bu_tile_blocks;
+    if (c.grab_strided <= 0) c.grab_strided = d.grab_strided;
     b->t.cfg = c;
     return BFS_OK;
 }
@@ -407,6 +427,8 @@ bfs_status bfs_batch_enact(bfs_batch *b, const int32_t *graph_ids, const int32_t
         s.src = sources[i];
         s.depth = reinterpret_cast<int *>(base + lay[i].depth);
         s.visited = reinterpret_cast<unsigned *>(base + lay[i].visited);
+        s.fbm[0] = reinterpret_cast<unsigned *>(base + lay[i].fbm0);
+        s.fbm[1] = reinterpret_cast<unsigned *>(base + lay[i].fbm1);
         s.q[0] = reinterpret_cast<int *>(base + lay[i].q0);
         s.q[1] = reinterpret_cast<int *>(base + lay[i].q1);
         s.cap = (unsigned) std::max(p->graphs[graph_ids[i]].nodes, 1);
@@ -436,7 +458,7 @@ bfs_status bfs_batch_enact(bfs_batch *b, const int32_t *graph_ids, const int32_t
             launches++;
         } else {
             const int levels = 8;
-            int grid = batch_grid(p, false, 0);
+            int grid = batch_grid(p, false, t.cfg.blocks_per_cu);
             st = t.capture(a, grid, levels);
             if (st != BFS_OK) break;
             while (true) {
@@ -464,6 +486,7 @@ bfs_status bfs_batch_enact(bfs_batch *b, const int32_t *graph_ids, const int32_t
     BatchControl hc;
     HIP_OK(hipMemcpy(&hc, t.d_ctl, sizeof(BatchControl), hipMemcpyDeviceToHost), "read batch control");
     b->nq = nq;
+    b->last = hc;
     b->latency_ms.assign(nq, 0.0);
     clock0 = ~0ull;
     for (int i = 0; i < nq; i++) clock0 = std::min(clock0, t.h_slots[i].t_begin);
@@ -501,6 +524,21 @@ bfs_status bfs_batch_extract_depths(const bfs_batch *b, int32_t query, int32_t *
     HIP_OK(hipMemcpy(h_depths, b->t.h_slots[query].depth, sizeof(int) * (size_t) b->nodes_of[query],
                      hipMemcpyDeviceToHost),
            "copy depths");
+    return BFS_OK;
+}
+
+bfs_status bfs_batch_level_log(const bfs_batch *b, int32_t max_levels, uint32_t *td_tiles, uint32_t *bu_tiles,
+                               double *start_us, int32_t *levels)
+{
+    if (!b || !levels || max_levels < 0) return host_fail(BFS_ERR_INVALID_ARG, "invalid argument");
+    const BatchControl &h = b->last;
+    int k = 0;
+    for (; k < kLevelLog && k < max_levels && k < h.level; k++) {
+        if (td_tiles) td_tiles[k] = h.level_td_tiles[k];
+        if (bu_tiles) bu_tiles[k] = h.level_bu_tiles[k];
+        if (start_us) start_us[k] = 1000.0 * (double) (h.level_clock[k] - h.level_clock[0]) / b->pool->wall_khz;
+    }
+    *levels = k;
     return BFS_OK;
 }
 
@@ -600,7 +638,7 @@ void bfs_server::run()
 {
     (void) hipSetDevice(pool->device);
     BatchArgs a = t.args(d_table, cfg.max_slots, cfg.direction_optimizing ? 1 : 0);
-    int grid = batch_grid(pool, false, 0);
+    int grid = batch_grid(pool, false, t.cfg.blocks_per_cu);
     bfs_status st = t.capture(a, grid, cfg.levels_per_tick);
     std::vector<int> admitted;
     auto busy_start = std::chrono::steady_clock::now();
@@ -644,6 +682,8 @@ void bfs_server::run()
                 bs.src = r.src;
                 bs.depth = reinterpret_cast<int *>(base + lay.depth);
                 bs.visited = reinterpret_cast<unsigned *>(base + lay.visited);
+                bs.fbm[0] = reinterpret_cast<unsigned *>(base + lay.fbm0);
+                bs.fbm[1] = reinterpret_cast<unsigned *>(base + lay.fbm1);
                 bs.q[0] = reinterpret_cast<int *>(base + lay.q0);
                 bs.q[1] = reinterpret_cast<int *>(base + lay.q1);
                 bs.cap = (unsigned) std::max(g.nodes, 1);

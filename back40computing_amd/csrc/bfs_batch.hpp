@@ -25,6 +25,7 @@ struct alignas(128) BatchSlot {
     int next_phase, ran;      // direction chosen for the level in flight, and whether it had work
     int *depth;
     unsigned *visited;
+    unsigned *fbm[2];         // frontier bitmaps of bottom-up levels, by iteration & 1
     int *q[2];
     unsigned cap;             // queue capacity (elements)
     unsigned qlen[3];
@@ -39,8 +40,16 @@ struct alignas(256) BatchControl {
     int pad0;
     unsigned long long t_start;
     unsigned bar_pad[26];
+    unsigned long long hub_alloc[2];  // grid tier: (entries << 32) | chunks, by level & 1
+    unsigned hub_steal[2];            // grid tier: chunk counter, by level & 1
+    unsigned pad1[28];
     // grid barrier state (same layout discipline as Control)
     Control bar;
+    // per-level record: wall clock at the level start, top-down and
+    // bottom-up tiles planned
+    unsigned long long level_clock[kLevelLog];
+    unsigned level_td_tiles[kLevelLog];
+    unsigned level_bu_tiles[kLevelLog];
 };
 
 struct BatchArgs {
@@ -51,6 +60,12 @@ struct BatchArgs {
     int wave_t, cta_t;
     int direction;            // 0: top-down only, 1: direction-optimizing
     int do_alpha, do_beta;
+    HubEntry *hubs[2];        // grid-tier rows of all queries, by level & 1
+    unsigned hub_cap;
+    int grid_t;               // rows at least this long go to the grid tier
+    int grab_max;             // most tiles taken per tile-counter atomic
+    int bu_tile;              // kBlock-vertex blocks per bottom-up tile
+    int strided;              // grab layout: tiles gridDim apart, else consecutive
 };
 
 }  // namespace b40c_hip

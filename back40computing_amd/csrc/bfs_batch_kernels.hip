@@ -29,10 +29,10 @@ struct BatchLds {
     int wave_tot[kWaves];
 };
 
-__device__ __forceinline__ unsigned next_item_batch(unsigned *counter, Lds<false> &lds)
+__device__ __forceinline__ unsigned next_item_batch(unsigned *counter, Lds<false> &lds, unsigned k = 1u)
 {
     __syncthreads();
-    if (threadIdx.x == 0) lds.tile = add_agent(counter, 1u);
+    if (threadIdx.x == 0) lds.tile = add_agent(counter, k);
     __syncthreads();
     return lds.tile;
 }
@@ -97,7 +97,8 @@ __device__ int batch_plan(const BatchArgs &a, BatchLds &L, int level)
             if (ld_agent((const unsigned *) &sl.overflow)) n = 0;
             if (n > 0) {
                 dir = batch_direction(a, sl, g, L.phase[tid], step, (unsigned) n);
-                tiles = dir == kPhaseBottomUp ? (g.nodes + kBlock - 1) / kBlock : (n + kBlock - 1) / kBlock;
+                const int bu = kBlock * a.bu_tile;
+                tiles = dir == kPhaseBottomUp ? (g.nodes + bu - 1) / bu : (n + kBlock - 1) / kBlock;
                 if (blockIdx.x == 0) {
                     st_agent(&sl.qlen[(step + 2) % 3], 0u);
                     __hip_atomic_store(&sl.fedges[(step + 2) % 3], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -117,11 +118,22 @@ __device__ int batch_plan(const BatchArgs &a, BatchLds &L, int level)
             sl.ran = n > 0;
         }
     }
-    if (blockIdx.x == 0 && tid == 0) st_agent(&a.ctl->steal[(level + 1) & 1], 0u);
+    if (blockIdx.x == 0 && tid == 0) {
+        st_agent(&a.ctl->steal[(level + 1) & 1], 0u);
+        st_agent(&a.ctl->hub_steal[(level + 1) & 1], 0u);
+        __hip_atomic_store(&a.ctl->hub_alloc[(level + 1) & 1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     int total = 0;
+    int bu_total = 0;
+    (void) block_exclusive_sum(tid < a.nslots && L.dir[tid] == kPhaseBottomUp ? tiles : 0, L.wave_tot, &bu_total);
     int excl = block_exclusive_sum(tiles, L.wave_tot, &total);
     if (tid < a.nslots) L.prefix[tid] = excl;
     if (tid == 0) L.prefix[a.nslots] = total;
+    if (blockIdx.x == 0 && tid == 0 && level < kLevelLog) {
+        a.ctl->level_clock[level] = wall_clock64();
+        a.ctl->level_td_tiles[level] = (unsigned) (total - bu_total);
+        a.ctl->level_bu_tiles[level] = (unsigned) bu_total;
+    }
     __syncthreads();
     return total;
 }
@@ -138,25 +150,71 @@ __device__ __forceinline__ void slot_edges(BatchSlot &sl, int step, unsigned fe)
     }
 }
 
-// One top-down (expand-contract) tile of query q.
-__device__ void batch_td_tile(const BatchArgs &a, BatchLds &L, int q, unsigned local)
+// Per-wave output of the query whose tiles the workgroup is processing: the
+// LDS-staged queue and the edge count stay open across consecutive tiles of
+// the same query and are flushed when the query changes or the level ends,
+// so each query's counters see few atomics however many tiles it has.
+struct SlotOut {
+    WaveQueue<false> wq;
+    unsigned fe;
+    int q;
+    int step;
+};
+
+__device__ __forceinline__ void out_close(SlotOut &o, const BatchArgs &a)
+{
+    if (o.q < 0) return;
+    o.wq.flush();
+    slot_edges(a.slots[o.q], o.step, o.fe);
+    o.q = -1;
+}
+
+__device__ __forceinline__ void out_open(SlotOut &o, const BatchArgs &a, BatchLds &L, int q)
+{
+    if (o.q == q) return;
+    out_close(o, a);
+    BatchSlot &sl = a.slots[q];
+    const int step = L.step[q];
+    o.wq.buf = L.base.wq[threadIdx.x / kWave];
+    o.wq.pbuf = nullptr;
+    o.wq.count = 0;
+    o.wq.out = sl.q[(step + 1) & 1];
+    o.wq.pout = nullptr;
+    o.wq.out_len = &sl.qlen[(step + 1) % 3];
+    o.wq.cap = sl.cap;
+    o.wq.overflow = (unsigned *) &sl.overflow;
+    o.fe = 0;
+    o.q = q;
+    o.step = step;
+}
+
+// Top-down edge visit of query q: exact claim, label, frontier edge count,
+// append to the query's next frontier.
+struct TdEdge {
+    BatchSlot &sl;
+    const BatchGraph &g;
+    int next_depth;
+    SlotOut &o;
+    __device__ __forceinline__ void operator()(bool ok, int u, int par) const
+    {
+        bool fresh = ok && claim_exact(sl.visited, u);
+        if (fresh) {
+            sl.depth[u] = next_depth;
+            o.fe += (unsigned) (g.row[u + 1] - g.row[u]);
+        }
+        o.wq.push(fresh, u, par);
+    }
+};
+
+// One top-down (expand-contract) tile of query q.  Rows of at least grid_t
+// neighbours are published to the batch-wide grid-tier list (tagged with q).
+__device__ void batch_td_tile(const BatchArgs &a, BatchLds &L, int q, unsigned local, SlotOut &o, int level)
 {
     BatchSlot &sl = a.slots[q];
     const BatchGraph g = a.graphs[sl.graph];
-    const int step = L.step[q], next_depth = L.it[q] + 1;
+    const int step = L.step[q];
     const unsigned n = (unsigned) L.n[q];
-    const int wave = threadIdx.x / kWave;
-    WaveQueue<false> wq;
-    wq.buf = L.base.wq[wave];
-    wq.pbuf = nullptr;
-    wq.count = 0;
-    wq.out = sl.q[(step + 1) & 1];
-    wq.pout = nullptr;
-    wq.out_len = &sl.qlen[(step + 1) % 3];
-    wq.cap = sl.cap;
-    wq.overflow = (unsigned *) &sl.overflow;
-    HubList nohub{nullptr, 0, nullptr, 0x7fffffff};
-    unsigned fe = 0;
+    HubList hub{a.hubs[level & 1], a.hub_cap, &a.ctl->hub_alloc[level & 1], a.grid_t, q};
     unsigned i = local * kBlock + threadIdx.x;
     int v = -1, beg = 0, end = 0;
     if (i < n) {
@@ -164,37 +222,87 @@ __device__ void batch_td_tile(const BatchArgs &a, BatchLds &L, int q, unsigned l
         beg = g.row[v];
         end = g.row[v + 1];
     }
-    gather_rows<false>(beg, end, v, g.col, a.wave_t, a.cta_t, nohub, L.base, [&](bool ok, int u, int par) {
-        bool fresh = ok && claim_exact(sl.visited, u);
-        if (fresh) {
-            sl.depth[u] = next_depth;
-            fe += (unsigned) (g.row[u + 1] - g.row[u]);
-        }
-        wq.push(fresh, u, par);
-    });
-    wq.flush();
-    slot_edges(sl, step, fe);
+    gather_rows<false>(beg, end, v, g.col, a.wave_t, a.cta_t, hub, L.base, TdEdge{sl, g, L.it[q] + 1, o});
 }
 
-// One bottom-up tile of query q: vertices [local*256, local*256+256).  The
-// frontier test is depth[u] == current depth (a vertex labelled during this
-// level holds depth + 1, so it never passes); each wave owns two bitmap words.
-__device__ void batch_bu_tile(const BatchArgs &a, BatchLds &L, int q, unsigned local)
+// Grid-tier phase of a level: every workgroup takes kHubChunk-edge chunks of
+// the rows the top-down tiles published.  Follows a grid barrier (or kernel
+// boundary) after the tiles.
+__device__ void batch_hub_phase(const BatchArgs &a, BatchLds &L, int level)
+{
+    unsigned long long al = ld64(&a.ctl->hub_alloc[level & 1]);
+    unsigned entries = (unsigned) (al >> 32);
+    if (entries > a.hub_cap) entries = a.hub_cap;
+    if (entries == 0) return;
+    const HubEntry *list = a.hubs[level & 1];
+    HubEntry last = list[entries - 1];
+    unsigned total = last.first_chunk + (unsigned) ((last.end - last.beg + kHubChunk - 1) / kHubChunk);
+    SlotOut o;
+    o.q = -1;
+    while (true) {
+        unsigned ch = next_item_batch(&a.ctl->hub_steal[level & 1], L.base);
+        if (ch >= total) break;
+        if (threadIdx.x == 0) {
+            unsigned lo = 0, hi = entries - 1;
+            while (lo < hi) {
+                unsigned mid = (lo + hi + 1) >> 1;
+                if (list[mid].first_chunk <= ch) lo = mid;
+                else hi = mid - 1;
+            }
+            HubEntry e = list[lo];
+            int b = e.beg + (int) (ch - e.first_chunk) * kHubChunk;
+            L.base.cta_beg = b;
+            L.base.cta_end = min(e.end, b + kHubChunk);
+            L.base.cta_par = e.par;
+            L.base.iteration = e.tag;
+        }
+        __syncthreads();
+        const int b = L.base.cta_beg, e = L.base.cta_end, pv = L.base.cta_par, q = L.base.iteration;
+        out_open(o, a, L, q);
+        BatchSlot &sl = a.slots[q];
+        const BatchGraph g = a.graphs[sl.graph];
+        TdEdge edge{sl, g, L.it[q] + 1, o};
+        for (int k = b; k < e; k += kBlock) {
+            int idx = k + threadIdx.x;
+            bool ok = idx < e;
+            int u = ok ? __builtin_nontemporal_load(g.col + idx) : -1;
+            edge(ok, u, pv);
+        }
+    }
+    out_close(o, a);
+}
+
+__device__ __forceinline__ bool batch_hubs_pending(const BatchArgs &a, int level)
+{
+    return ld64(&a.ctl->hub_alloc[level & 1]) != 0ull;
+}
+
+// One kBlock-vertex block of a bottom-up tile of query q.  The
+// frontier test is depth[u] == current depth after a top-down level (a vertex
+// labelled during this level holds depth + 1, so it never passes) and the
+// frontier bitmap after a bottom-up level; each wave owns two words of the
+// visited and next-frontier bitmaps.
+__device__ void batch_bu_block(const BatchArgs &a, BatchLds &L, int q, unsigned local, SlotOut &o)
 {
     BatchSlot &sl = a.slots[q];
     const BatchGraph g = a.graphs[sl.graph];
-    const int step = L.step[q], d = L.it[q];
-    const int lane = lane_id(), wave = threadIdx.x / kWave;
-    WaveQueue<false> wq;
-    wq.buf = L.base.wq[wave];
-    wq.pbuf = nullptr;
-    wq.count = 0;
-    wq.out = sl.q[(step + 1) & 1];
-    wq.pout = nullptr;
-    wq.out_len = &sl.qlen[(step + 1) % 3];
-    wq.cap = sl.cap;
-    wq.overflow = (unsigned *) &sl.overflow;
-    unsigned fe = 0;
+    const int d = L.it[q];
+    const int lane = lane_id();
+    WaveQueue<false> &wq = o.wq;
+    unsigned &fe = o.fe;
+    // After a bottom-up level the current frontier is the bitmap that level
+    // wrote (L2-resident); after a top-down level it is read from the labels.
+    const bool use_bitmap = L.phase[q] == kPhaseBottomUp;
+    const unsigned *cur = sl.fbm[d & 1];
+    unsigned *nxt = sl.fbm[(d + 1) & 1];
+    // After a top-down level the visited bitmap (small, cache-resident) is
+    // tested first, so only visited neighbours cost a label read; a vertex
+    // labelled during this level holds depth d + 1 (or still -1 if its
+    // label store is not yet visible) and fails the label test either way.
+    auto in_frontier = [&](int u) {
+        if (use_bitmap) return ((cur[u >> 5] >> (u & 31)) & 1u) != 0;
+        return ((sl.visited[u >> 5] >> (u & 31)) & 1u) != 0 && sl.depth[u] == d;
+    };
     int v = (int) (local * kBlock + threadIdx.x);
     unsigned w = (unsigned) v >> 5;
     bool valid = v < g.nodes;
@@ -212,10 +320,10 @@ __device__ void batch_bu_tile(const BatchArgs &a, BatchLds &L, int q, unsigned l
             int u1 = e + 1 < end ? g.rcol[e + 1] : -1;
             int u2 = e + 2 < end ? g.rcol[e + 2] : -1;
             int u3 = e + 3 < end ? g.rcol[e + 3] : -1;
-            if (sl.depth[u0] == d) { parent = u0; break; }
-            if (u1 >= 0 && sl.depth[u1] == d) { parent = u1; break; }
-            if (u2 >= 0 && sl.depth[u2] == d) { parent = u2; break; }
-            if (u3 >= 0 && sl.depth[u3] == d) { parent = u3; break; }
+            if (in_frontier(u0)) { parent = u0; break; }
+            if (u1 >= 0 && in_frontier(u1)) { parent = u1; break; }
+            if (u2 >= 0 && in_frontier(u2)) { parent = u2; break; }
+            if (u3 >= 0 && in_frontier(u3)) { parent = u3; break; }
         }
     }
     uint64_t lm = ballot(todo && len >= 64);
@@ -227,7 +335,7 @@ __device__ void batch_bu_tile(const BatchArgs &a, BatchLds &L, int q, unsigned l
         for (int k = b; k < e; k += kWave) {
             int idx = k + lane;
             int u = idx < e ? g.rcol[idx] : -1;
-            uint64_t hm = ballot(u >= 0 && sl.depth[u] == d);
+            uint64_t hm = ballot(u >= 0 && in_frontier(u));
             if (hm) {
                 found = readlane(u, uniform(__builtin_ctzll(hm)));
                 break;
@@ -244,30 +352,60 @@ __device__ void batch_bu_tile(const BatchArgs &a, BatchLds &L, int q, unsigned l
     if ((lane & 31) == 0 && valid) {
         unsigned bits = (unsigned) (lane == 0 ? gm : gm >> 32);
         if (bits) sl.visited[w] = vis | bits;
+        nxt[w] = bits;  // whole word: also clears it
     }
     wq.push(got, v, parent);
-    wq.flush();
-    slot_edges(sl, step, fe);
+}
+
+// One bottom-up tile: bu_tile consecutive kBlock-vertex blocks.
+__device__ void batch_bu_tile(const BatchArgs &a, BatchLds &L, int q, unsigned local, SlotOut &o)
+{
+    const unsigned nblocks = (unsigned) (a.graphs[a.slots[q].graph].nodes + kBlock - 1) / kBlock;
+    const unsigned first = local * (unsigned) a.bu_tile;
+    const unsigned last = min(nblocks, first + (unsigned) a.bu_tile);
+    for (unsigned b = first; b < last; b++) batch_bu_block(a, L, q, b, o);
 }
 
 // Runs the tiles of one planned level.
 __device__ void batch_tiles(const BatchArgs &a, BatchLds &L, int level, int total)
 {
+    SlotOut o;
+    o.q = -1;
+    // Grabs: one counter atomic hands a workgroup `grab` tiles, so the
+    // single counter address (whose device-scope atomics serialise) is not
+    // the bottleneck of levels with many tiles.  Strided: counter value i
+    // takes tiles i0, i0 + W, ... (W = gridDim.x), so W consecutive grabs
+    // cover W * grab consecutive tiles and the tiles in flight across the
+    // grid stay a window about W wide.  Consecutive: counter value t0 takes
+    // [t0, t0 + grab).
+    const unsigned W = gridDim.x;
+    const unsigned grab = (unsigned) max(1, min(a.grab_max, total / ((int) W * 8)));
+    const unsigned step = a.strided ? W : 1u;
     while (true) {
-        unsigned t = next_item_batch(&a.ctl->steal[level & 1], L.base);
-        if (t >= (unsigned) total) break;
-        // last query whose first tile is <= t (queries without tiles repeat
-        // the prefix value and are skipped by taking the last match)
-        int lo = 0, hi = a.nslots - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if ((unsigned) L.prefix[mid] <= t) lo = mid;
-            else hi = mid - 1;
+        unsigned t0;
+        if (a.strided) {
+            const unsigned i = next_item_batch(&a.ctl->steal[level & 1], L.base);
+            t0 = (i / W) * W * grab + i % W;
+        } else {
+            t0 = next_item_batch(&a.ctl->steal[level & 1], L.base, grab);
         }
-        unsigned local = t - (unsigned) L.prefix[lo];
-        if (L.dir[lo] == kPhaseBottomUp) batch_bu_tile(a, L, lo, local);
-        else batch_td_tile(a, L, lo, local);
+        if (t0 >= (unsigned) total) break;
+        for (unsigned t = t0, k = 0; k < grab && t < (unsigned) total; k++, t += step) {
+            // last query whose first tile is <= t (queries without tiles
+            // repeat the prefix value and are skipped by taking the last match)
+            int lo = 0, hi = a.nslots - 1;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if ((unsigned) L.prefix[mid] <= t) lo = mid;
+                else hi = mid - 1;
+            }
+            unsigned local = t - (unsigned) L.prefix[lo];
+            out_open(o, a, L, lo);
+            if (L.dir[lo] == kPhaseBottomUp) batch_bu_tile(a, L, lo, local, o);
+            else batch_td_tile(a, L, lo, local, o, level);
+        }
     }
+    out_close(o, a);
 }
 
 __device__ void batch_load_cursors(const BatchArgs &a, BatchLds &L)
@@ -296,6 +434,10 @@ __global__ __launch_bounds__(kBlock) void batch_persistent_kernel(BatchArgs a)
         if (total == 0) break;
         batch_tiles(a, L, level, total);
         grid_barrier(&a.ctl->bar);
+        if (batch_hubs_pending(a, level)) {
+            batch_hub_phase(a, L, level);
+            grid_barrier(&a.ctl->bar);
+        }
         // every workgroup advances its own copy of the cursors identically
         int tid = threadIdx.x;
         if (tid < a.nslots && L.n[tid] > 0) {
@@ -327,6 +469,16 @@ __global__ __launch_bounds__(kBlock) void batch_level_kernel(BatchArgs a)
     int total = batch_plan(a, L, level);
     if (total == 0) return;
     batch_tiles(a, L, level, total);
+}
+
+// Grid-tier chunks of a level run as their own launch.
+__global__ __launch_bounds__(kBlock) void batch_hub_kernel(BatchArgs a)
+{
+    __shared__ BatchLds L;
+    int level = ld_agent(&a.ctl->level);
+    if (!batch_hubs_pending(a, level)) return;
+    batch_load_cursors(a, L);
+    batch_hub_phase(a, L, level);
 }
 
 // Single workgroup: moves every query that had a frontier one step on and
@@ -412,6 +564,7 @@ hipError_t launch_batch_persistent(const BatchArgs &a, int grid, hipStream_t st)
 hipError_t launch_batch_level(const BatchArgs &a, int grid, int *alive_out, hipStream_t st)
 {
     hipLaunchKernelGGL(batch_level_kernel, dim3(grid), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL(batch_hub_kernel, dim3(grid), dim3(kBlock), 0, st, a);
     hipLaunchKernelGGL(batch_advance_kernel, dim3(1), dim3(kBlock), 0, st, a, alive_out);
     return hipGetLastError();
 }

@@ -55,7 +55,10 @@ const bfs_config kDefaults = {
     /*grid_gather_threshold*/ 8192,
     /*do_alpha*/ 14,
     /*do_beta*/ 24,
-    {0, 0, 0, 0, 0, 0}};
+    /*tile_grab_max*/ 32,
+    /*bu_tile_blocks*/ 1,
+    /*grab_strided*/ 1,
+    {0, 0, 0}};
 
 constexpr unsigned kHubCapacity = 1u << 16;  // grid-tier entries per step list
 
@@ -138,6 +141,7 @@ SearchArgs search_args(const bfs_problem *p, int grid)
     s.wave_t = p->cfg.wave_gather_threshold;
     s.cta_t = p->cfg.cta_gather_threshold;
     s.grid_t = p->cfg.grid_gather_threshold > 0 ? p->cfg.grid_gather_threshold : 0x7fffffff;
+    s.grab_max = p->cfg.tile_grab_max > 0 ? p->cfg.tile_grab_max : 1;
     s.fast_t = p->cfg.fast_path_threshold;
     s.sat_t = p->cfg.saturation_threshold > 0 ? p->cfg.saturation_threshold
               : (p->cfg.saturation_threshold < 0 ? -1 : grid * 256);

@@ -65,6 +65,7 @@ struct alignas(256) Control {
 struct HubEntry {
     int beg, end, par;
     unsigned first_chunk;
+    int tag;   // owner of the row (the query, in batched searches)
 };
 
 enum ExitReason { kExitDone = 0, kExitSaturated = 1, kExitOverflow = 2 };
@@ -268,6 +269,7 @@ struct HubList {
     unsigned cap;
     unsigned long long *alloc;
     int grid_t;        // rows at least this long go to the grid tier
+    int tag;           // stored with every published row
 };
 
 template <bool PRED, typename Fn>
@@ -288,7 +290,7 @@ __device__ __forceinline__ void gather_rows(int beg, int end, int par, const int
                                                         __HIP_MEMORY_SCOPE_AGENT);
         unsigned slot = (unsigned) (old >> 32);
         if (slot < hub.cap) {
-            hub.entries[slot] = HubEntry{beg, end, par, (unsigned) old};
+            hub.entries[slot] = HubEntry{beg, end, par, (unsigned) old, hub.tag};
             len = 0;
         }
     }

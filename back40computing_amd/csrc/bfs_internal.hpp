@@ -46,6 +46,7 @@ struct SearchArgs {
     int do_alpha, do_beta; // top-down -> bottom-up when m_f * alpha > m_u; back when n_f * beta < V
     int wave_t, cta_t, grid_t, fast_t, sat_t;
     int instrument;
+    int grab_max;          // most tiles taken per tile-counter atomic
 };
 
 // Loop position: step index (selects queues and counters), BFS iteration

@@ -34,12 +34,41 @@ __device__ __forceinline__ void init_lds(Lds<PRED> &lds)
 
 // Dynamic work assignment: one agent-scope atomic per tile (or chunk).
 template <bool PRED>
-__device__ __forceinline__ unsigned next_item(unsigned *counter, Lds<PRED> &lds)
+__device__ __forceinline__ unsigned next_item(unsigned *counter, Lds<PRED> &lds, unsigned k = 1u)
 {
     __syncthreads();
-    if (threadIdx.x == 0) lds.tile = add_agent(counter, 1u);
+    if (threadIdx.x == 0) lds.tile = add_agent(counter, k);
     __syncthreads();
     return lds.tile;
+}
+
+// Guided grabs of kBlock-item tiles: one counter atomic hands a workgroup
+// `grab` consecutive tiles, about eight grabs per workgroup when a phase has
+// many tiles, so the single counter address (whose device-scope atomics
+// serialise) stays off the critical path of large levels.
+struct TileGrab {
+    unsigned next, end, grab;
+};
+
+__device__ __forceinline__ TileGrab tile_grab(const SearchArgs &s, unsigned items)
+{
+    unsigned tiles = (items + kBlock - 1) / kBlock;
+    unsigned g = tiles / (gridDim.x * 8u);
+    g = max(1u, min((unsigned) s.grab_max, g));
+    return TileGrab{0u, 0u, g};
+}
+
+// Next tile index of the workgroup (workgroup-uniform).
+template <bool PRED>
+__device__ __forceinline__ unsigned next_tile(unsigned *counter, Lds<PRED> &lds, TileGrab &tg)
+{
+    if (tg.next == tg.end) {
+        tg.next = next_item(counter, lds, tg.grab);
+        tg.end = tg.next + tg.grab;
+    } else {
+        __syncthreads();  // same LDS reuse fence as a fresh grab
+    }
+    return tg.next++;
 }
 
 // LDS history hash: true if u may be new to this workgroup.  A hit means
@@ -81,6 +110,7 @@ __device__ __forceinline__ HubList hub_list(const SearchArgs &s, int step, bool 
     h.cap = s.hub_cap;
     h.alloc = &s.ctl->hub_alloc[step & 1];
     h.grid_t = enabled ? s.grid_t : 0x7fffffff;
+    h.tag = 0;
     return h;
 }
 
@@ -121,8 +151,9 @@ template <bool PRED, typename RowOf, typename Edge>
 __device__ __forceinline__ void tile_phase(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n,
                                            const HubList &hub, Lds<PRED> &lds, RowOf &&row_of, Edge &&edge)
 {
+    TileGrab tg = tile_grab(s, n);
     while (true) {
-        unsigned base = next_item(&s.ctl->steal[c.step & 1], lds) * kBlock;
+        unsigned base = next_tile(&s.ctl->steal[c.step & 1], lds, tg) * kBlock;
         if (base >= n) break;
         unsigned i = base + threadIdx.x;
         int beg = 0, end = 0, par = -1;
@@ -234,8 +265,9 @@ __device__ void body_bottom_up(const GraphArgs &g, const SearchArgs &s, Cursor c
     auto in_frontier = [&](int u) { return ((cur[u >> 5] >> (u & 31)) & 1u) != 0; };
     bool labeled = false;
     unsigned fe = 0;
+    TileGrab tg = tile_grab(s, (unsigned) g.nodes);
     while (true) {
-        unsigned base = next_item(&s.ctl->steal[c.step & 1], lds) * kBlock;
+        unsigned base = next_tile(&s.ctl->steal[c.step & 1], lds, tg) * kBlock;
         if (base >= (unsigned) g.nodes) break;
         int v = (int) (base + threadIdx.x);
         unsigned w = (unsigned) v >> 5;
@@ -358,8 +390,9 @@ __device__ void body_contract(const GraphArgs &g, const SearchArgs &s, Cursor c,
     const int *in = s.q[c.step & 1];
     const int *pin = s.pq[c.step & 1];
     bool labeled = false;
+    TileGrab tg = tile_grab(s, n);
     while (true) {
-        unsigned base = next_item(&s.ctl->steal[c.step & 1], lds) * kBlock;
+        unsigned base = next_tile(&s.ctl->steal[c.step & 1], lds, tg) * kBlock;
         if (base >= n) break;
         unsigned i = base + threadIdx.x;
         int u = -1, p = -1;
@@ -386,8 +419,9 @@ __device__ void body_filter(const GraphArgs &g, const SearchArgs &s, Cursor c, u
     WaveQueue<PRED> wq = make_queue<PRED>(s, c.step, lds);
     const int *in = s.q[c.step & 1];
     const int *pin = s.pq[c.step & 1];
+    TileGrab tg = tile_grab(s, n);
     while (true) {
-        unsigned base = next_item(&s.ctl->steal[c.step & 1], lds) * kBlock;
+        unsigned base = next_tile(&s.ctl->steal[c.step & 1], lds, tg) * kBlock;
         if (base >= n) break;
         unsigned i = base + threadIdx.x;
         int u = -1, p = -1;

@@ -106,7 +106,13 @@ typedef struct bfs_config {
                                        frontier edges * alpha > unexplored edges */
     int32_t do_beta;                /* direction-optimizing: back to top-down when
                                        frontier vertices * beta < V */
-    int32_t reserved[6];
+    int32_t tile_grab_max;          /* most 256-item tiles a workgroup takes per
+                                       tile-counter atomic; the grab is
+                                       tiles / (grid * 8) clamped to [1, this] */
+    int32_t bu_tile_blocks;         /* batched: 256-vertex blocks per bottom-up tile */
+    int32_t grab_strided;           /* batched: 1 = a grab's tiles are gridDim apart
+                                       (narrow in-flight window), 2 = consecutive */
+    int32_t reserved[3];
 } bfs_config;
 
 typedef struct bfs_stats {

@@ -63,6 +63,10 @@ bfs_status bfs_batch_enact(bfs_batch *b, const int32_t *graph_ids, const int32_t
 /* Completion time of every query of the last enact, ms from the loop start. */
 bfs_status bfs_batch_query_latency(const bfs_batch *b, double *ms);
 bfs_status bfs_batch_extract_depths(const bfs_batch *b, int32_t query, int32_t *h_depths);
+/* Per-level record of the last enact (first 64 levels): top-down and
+ * bottom-up tiles planned and the level start in microseconds. */
+bfs_status bfs_batch_level_log(const bfs_batch *b, int32_t max_levels, uint32_t *td_tiles, uint32_t *bu_tiles,
+                               double *start_us, int32_t *levels);
 void bfs_batch_destroy(bfs_batch *b);
 
 typedef struct bfs_server_config {
