@@ -165,6 +165,7 @@ def load():
     _sig(lib, "bfs_query_state_bytes", i64, i32)
     _sig(lib, "bfs_batch_create", ctypes.c_int, vp, i32, vpp)
     _sig(lib, "bfs_batch_set_config", ctypes.c_int, vp, ctypes.POINTER(BfsConfig))
+    _sig(lib, "bfs_batch_config_default", ctypes.c_int, ctypes.POINTER(BfsConfig))
     _sig(lib, "bfs_batch_enact", ctypes.c_int, vp, c_int32_p, c_int32_p, i32, i32, i32, ctypes.POINTER(BatchStats))
     _sig(lib, "bfs_batch_query_latency", ctypes.c_int, vp, ctypes.POINTER(dbl))
     _sig(lib, "bfs_batch_extract_depths", ctypes.c_int, vp, i32, c_int32_p)

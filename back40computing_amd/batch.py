@@ -55,7 +55,7 @@ class Batch:
 
     def set_config(self, **kw):
         cfg = _native.BfsConfig()
-        _native.load().bfs_config_default(ctypes.byref(cfg))
+        _native.load().bfs_batch_config_default(ctypes.byref(cfg))
         for k, v in kw.items():
             setattr(cfg, k, int(v))
         _check(_native.load().bfs_batch_set_config(self._h, ctypes.byref(cfg)), "bfs_batch_set_config")

@@ -130,7 +130,7 @@ struct SlotTable {
         memset(h_slots, 0, sizeof(BatchSlot) * slots);
         HIP_OK(hipMemcpy(d_slots, h_slots, sizeof(BatchSlot) * slots, hipMemcpyHostToDevice), "clear slots");
         HIP_OK(hipMemset(d_ctl, 0, sizeof(BatchControl)), "clear batch control");
-        bfs_config_default(&cfg);
+        bfs_batch_config_default(&cfg);
         return BFS_OK;
     }
 
@@ -259,6 +259,26 @@ extern "C" {
 
 int64_t bfs_query_state_bytes(int32_t nodes) { return (int64_t) slot_layout(nodes).bytes; }
 
+bfs_status bfs_batch_config_default(bfs_config *cfg)
+{
+    if (!cfg) return host_fail(BFS_ERR_INVALID_ARG, "null config");
+    bfs_config_default(cfg);
+    // Autotuned on the 256-query R-MAT pool (64 graphs, scale 14..20):
+    // bottom-up tiles of 64 blocks (16384 consecutive vertices per
+    // workgroup), four workgroups per CU, large grabs, an earlier grid tier
+    // and a later switch to bottom-up than the single-source default.
+    cfg->blocks_per_cu = 4;
+    cfg->bu_tile_blocks = 64;
+    cfg->tile_grab_max = 128;
+    cfg->grab_strided = 2;
+    cfg->grid_gather_threshold = 4096;
+    cfg->wave_gather_threshold = 64;
+    cfg->cta_gather_threshold = 1024;
+    cfg->do_alpha = 8;
+    cfg->do_beta = 48;
+    return BFS_OK;
+}
+
 bfs_status bfs_pool_create(int32_t device, bfs_graph_pool **out)
 {
     if (!out) return host_fail(BFS_ERR_INVALID_ARG, "null output");
@@ -374,7 +394,7 @@ bfs_status bfs_batch_set_config(bfs_batch *b, const bfs_config *cfg)
 {
     if (!b || !cfg) return host_fail(BFS_ERR_INVALID_ARG, "null argument");
     bfs_config c = *cfg, d;
-    bfs_config_default(&d);
+    bfs_batch_config_default(&d);
     if (c.wave_gather_threshold <= 0) c.wave_gather_threshold = d.wave_gather_threshold;
     if (c.cta_gather_threshold <= 0) c.cta_gather_threshold = d.cta_gather_threshold;
     if (c.do_alpha <= 0) c.do_alpha = d.do_alpha;

@@ -44,17 +44,20 @@ bfs_status hip_fail(hipError_t e, const char *what)
         if (e_ != hipSuccess) return hip_fail(e_, what); \
     } while (0)
 
+// Defaults: the autotuner's pick for direction-optimizing search on R-MAT
+// scale 22, edge factor 16 (back40computing_amd/autotune.py; its table is
+// back40computing_amd/tuned.json).
 const bfs_config kDefaults = {
     /*wave_gather_threshold*/ 64,
-    /*cta_gather_threshold*/ 1024,
+    /*cta_gather_threshold*/ 256,
     /*fast_path_threshold*/ 1024,
     /*saturation_threshold*/ 0,   // 0: persistent grid size * 256
     /*blocks_per_cu*/ 2,
     /*cull_mode*/ BFS_CULL_EXACT,
     /*instrument*/ 0,
     /*grid_gather_threshold*/ 8192,
-    /*do_alpha*/ 14,
-    /*do_beta*/ 24,
+    /*do_alpha*/ 24,
+    /*do_beta*/ 48,
     /*tile_grab_max*/ 32,
     /*bu_tile_blocks*/ 1,
     /*grab_strided*/ 1,

@@ -403,9 +403,18 @@ __device__ __forceinline__ void grid_barrier(Control *ctl)
             if (r == leaves - 1) {
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
                 __hip_atomic_store(&ctl->bar_root, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                for (unsigned l = 0; l < leaves; l++)
-                    __hip_atomic_store(&ctl->bar_release[l * 32], gen + 1, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+                // Each leaf's word moves one past its own value: launches
+                // with fewer than kBarrierFanIn workgroups leave the upper
+                // leaves behind, so the words need not agree, and writing
+                // this leaf's gen + 1 everywhere could hand a leaf that is
+                // one generation ahead its current value (its waiters would
+                // never see a change).  Nothing else writes these words
+                // while every workgroup waits here.
+                for (unsigned l = 0; l < leaves; l++) {
+                    unsigned *rw = &ctl->bar_release[l * 32];
+                    unsigned cur = __hip_atomic_load(rw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(rw, cur + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
                 last = true;
             }
         }

@@ -578,6 +578,13 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
             // Single-workgroup fast path: block 0 runs consecutive small
             // levels with workgroup barriers only (no grid tier: it alone
             // works); the rest of the grid waits at the next grid barrier.
+            // The entry barrier makes every workgroup finish reading this
+            // step's counters first: block 0's second fast step zeroes
+            // qlen / fedges of index step % 3, which a workgroup still on
+            // its way out of the previous barrier would otherwise read as
+            // an empty frontier and leave the loop, stranding the grid at
+            // the barrier below.
+            grid_barrier(s.ctl);
             if (blockIdx.x == 0) {
                 unsigned long long t0 = s.instrument ? wall_clock64() : 0;
                 while (true) {

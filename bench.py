@@ -72,7 +72,7 @@ def pct(a, q):
     return round(float(np.percentile(np.asarray(a), q)), 4) if len(a) else None
 
 
-def bench_batched(b40c, np, reps):
+def bench_batched(b40c, np, reps, tuned_batch, tuned_single):
     from back40computing_amd.batch import Batch, GraphPool
     from back40computing_amd.workloads import batched_rmat
 
@@ -85,6 +85,8 @@ def bench_batched(b40c, np, reps):
     out = {"workload": "64 R-MAT graphs scale U{14..20} ef U{8..32}, 256 queries",
            "pool_vertices": int(sum(g.nodes for g in graphs)), "pool_edges": int(sum(g.edges for g in graphs))}
     b = Batch(pool, max_queries=len(gids))
+    if tuned_batch:
+        b.set_config(**tuned_batch)
     for name, mode in [("persistent", Batch.PERSISTENT), ("graph", Batch.GRAPH)]:
         b.enact(gids, srcs, direction_optimizing=True, mode=mode)  # warm-up
         best = None
@@ -105,6 +107,8 @@ def bench_batched(b40c, np, reps):
     problems = []
     for g in graphs:
         p = b40c.CsrProblem().from_host_problem(g)
+        if tuned_single:
+            p.set_config(**tuned_single)
         p.reset(b40c.VERTEX_FRONTIERS, 1.0)
         problems.append(p)
     for q in range(8):
@@ -185,6 +189,8 @@ def main():
     ap.add_argument("--sources", type=int, default=64)
     ap.add_argument("--cpu-sample", type=int, default=12, help="sources timed on the CPU oracle (0: skip)")
     ap.add_argument("--config", default="", help="comma list of bfs_config overrides, e.g. blocks_per_cu=2")
+    ap.add_argument("--no-tuned", action="store_true",
+                    help="library defaults instead of the autotuned table (back40computing_amd/tuned.json)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched section")
     ap.add_argument("--batched-reps", type=int, default=5)
     ap.add_argument("--serving", action="store_true", help="add the serving sweep")
@@ -218,12 +224,15 @@ def main():
     fused = not args.iterative
 
     p = b40c.CsrProblem().from_host_problem(g)
+    from back40computing_amd import autotune
+    tuned_single = {} if args.no_tuned else autotune.tuned_config("single")
+    tuned_batch = {} if args.no_tuned else autotune.tuned_config("batch")
     overrides = {}
     for kv in filter(None, args.config.split(",")):
         k, v = kv.split("=")
         overrides[k.strip()] = int(v)
-    if overrides:
-        p.set_config(**overrides)
+    if tuned_single or overrides:
+        p.set_config(**{**tuned_single, **overrides})
     p.reset(lib.bfs_strategy_frontier_type(strategy), 1.3)
 
     # Untimed pass: traversed-edge counts per source and the labels kept for
@@ -304,7 +313,7 @@ def main():
 
     batched = None
     if not args.no_batched:
-        batched, bgraphs, bgids, bsrcs = bench_batched(b40c, np, args.batched_reps)
+        batched, bgraphs, bgids, bsrcs = bench_batched(b40c, np, args.batched_reps, tuned_batch, tuned_single)
         if cpu is not None:
             import oracle
             sample = range(0, len(bgids), 8)
@@ -334,7 +343,8 @@ def main():
         "config": {"workload": workload, "graph": "R-MAT a=0.57 b=0.19 c=0.19 d=0.05, symmetrised, "
                    "self-loops and duplicates removed, scrambled ids", "scale": args.scale,
                    "edge_factor": args.edge_factor, "vertices": int(len(deg)), "directed_edges": int(deg.sum()),
-                   "sources": len(srcs), "strategy": args.strategy, "fused": fused, "overrides": overrides},
+                   "sources": len(srcs), "strategy": args.strategy, "fused": fused, "overrides": overrides,
+                   "tuned": bool(tuned_single), "tuned_single": tuned_single, "tuned_batch": tuned_batch},
         "gteps_kernel_mean": round(float(per_src.mean()), 3),
         "gteps_kernel_hmean": round(float(len(per_src) / np.sum(1.0 / per_src)), 3),
         "kernel_ms_mean": round(float(kernel_ms.mean()), 4),

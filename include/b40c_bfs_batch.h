@@ -55,6 +55,10 @@ void bfs_pool_destroy(bfs_graph_pool *pool);
 int64_t bfs_query_state_bytes(int32_t nodes);
 
 bfs_status bfs_batch_create(bfs_graph_pool *pool, int32_t max_queries, bfs_batch **out);
+/* Library defaults for batches and servers (bfs_config_default with the
+ * batched tile shape and switch points).  Zero fields passed to
+ * bfs_batch_set_config take these values. */
+bfs_status bfs_batch_config_default(bfs_config *cfg);
 bfs_status bfs_batch_set_config(bfs_batch *b, const bfs_config *cfg);
 /* Runs nq (<= max_queries, <= 256) queries to completion.  mode 0: one
  * persistent launch; mode 1: hipGraph-captured per-level launches. */
