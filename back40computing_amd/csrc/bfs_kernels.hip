@@ -547,9 +547,12 @@ __device__ __forceinline__ bool hubs_pending(const SearchArgs &s, int step)
 // ---------------------------------------------------------------------------
 // Persistent level loop (cooperative launch).
 // ---------------------------------------------------------------------------
-template <bool PRED, bool EXACT>
-__global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchArgs s, Cursor c, int loop)
+// One instantiation per strategy (LOOP), so profiles attribute time to the
+// strategy and the level loop carries no dead strategy branches.
+template <bool PRED, bool EXACT, int LOOP>
+__global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchArgs s, Cursor c)
 {
+    constexpr int loop = LOOP;
     __shared__ Lds<PRED> lds;
     init_lds(lds);
     unsigned long long t_begin = s.instrument ? wall_clock64() : 0, busy = 0;
@@ -750,15 +753,20 @@ __global__ __launch_bounds__(kWave) void wave_selftest_kernel(const int *in, int
 namespace {
 
 template <bool PRED, bool EXACT>
-const void *persistent_ptr()
+const void *persistent_ptr(int loop)
 {
-    return reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT>);
+    switch (loop) {
+    case kLoopEC: return reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopEC>);
+    case kLoopCE: return reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopCE>);
+    case kLoopTP: return reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopTP>);
+    default: return reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopDO>);
+    }
 }
 
-const void *persistent_select(bool pred, bool exact)
+const void *persistent_select(int loop, bool pred, bool exact)
 {
-    if (pred) return exact ? persistent_ptr<true, true>() : persistent_ptr<true, false>();
-    return exact ? persistent_ptr<false, true>() : persistent_ptr<false, false>();
+    if (pred) return exact ? persistent_ptr<true, true>(loop) : persistent_ptr<true, false>(loop);
+    return exact ? persistent_ptr<false, true>(loop) : persistent_ptr<false, false>(loop);
 }
 
 template <bool PRED, bool EXACT, int PHASE>
@@ -830,16 +838,15 @@ hipError_t launch_persistent(int loop, bool pred, bool exact, const GraphArgs &g
     GraphArgs ga = g;
     SearchArgs sa = s;
     Cursor ca = c;
-    int la = loop;
-    void *args[] = {&ga, &sa, &ca, &la};
-    return hipLaunchCooperativeKernel(persistent_select(pred, exact), dim3(grid), dim3(kBlock), args, 0, st);
+    void *args[] = {&ga, &sa, &ca};
+    return hipLaunchCooperativeKernel(persistent_select(loop, pred, exact), dim3(grid), dim3(kBlock), args, 0, st);
 }
 
 int persistent_max_blocks_per_cu(int loop, bool pred, bool exact)
 {
-    (void) loop;
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, persistent_select(pred, exact), kBlock, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, persistent_select(loop, pred, exact), kBlock, 0) !=
+        hipSuccess)
         return 0;
     return blocks;
 }

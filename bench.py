@@ -273,8 +273,8 @@ def main():
     workload = "rmat%d_ef%d_%dsrc" % (args.scale, args.edge_factor, len(srcs))
     traffic = load_traffic(workload)
     traffic_bytes = None
-    if traffic and traffic.get("strategy") == args.strategy and traffic.get("fused", True) == fused:
-        traffic_bytes = traffic.get("hbm_bytes_per_launch")
+    if traffic and fused and args.strategy in traffic.get("strategies", {}):
+        traffic_bytes = traffic["strategies"][args.strategy]["hbm_bytes_per_launch"]
 
     # Top-down expand-contract over the same sources, for reference.
     top_down = None
