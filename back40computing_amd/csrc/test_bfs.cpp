@@ -100,7 +100,8 @@ void usage()
         "         [--strategy=<s>[,<s>]*] [--iterative] [--max-ctas=<n>] [--quick] [--v]\n\n"
         "Graph types: grid2d <width> | grid3d <width> | rmat <scale> <edge-factor> |\n"
         "             er <n> <m> | random <n> <m> [--undirected]\n"
-        "Strategies: 0 expand-contract, 1 contract-expand, 2 two-phase, 3 hybrid (default 3)\n\n");
+        "Strategies: 0 expand-contract, 1 contract-expand, 2 two-phase, 3 hybrid (default 3),\n"
+        "            4 direction-optimizing (addition)\n\n");
 }
 
 }  // namespace
@@ -164,7 +165,7 @@ int main(int argc, char **argv)
         problem.device = std::atoi(args.get("device", "0").c_str());
         if (problem.FromHostProblem(false, g.nodes, (int) g.edges, g.column_indices, g.row_offsets, 1)) return 1;
         std::vector<int> ref, labels(g.nodes), depths(g.nodes);
-        Statistic cpu_rate, rates[4];
+        Statistic cpu_rate, rates[5];
         for (int it = 0; it <= iterations; it++) {
             int src = sources[it];
             std::printf("---------------------------------------------------------------\n");
@@ -182,6 +183,7 @@ int main(int argc, char **argv)
                 EnactorContractExpand<false> ce(verbose);
                 EnactorTwoPhase<false> tp(verbose);
                 EnactorHybrid<false> hy(verbose);
+                EnactorDirectionOptimizing<false> dopt(verbose);
                 EnactorBase *base = nullptr;
                 bfs_status st = BFS_OK;
                 const char *name = "";
@@ -209,6 +211,13 @@ int main(int argc, char **argv)
                     if ((st = problem.Reset(hy.GetFrontierType(), qs))) break;
                     st = hy.EnactSearch(problem, src, max_ctas);
                     base = &hy;
+                    break;
+                case 4:
+                    name = "Direction-optimizing GPU BFS";
+                    if ((st = problem.Reset(dopt.GetFrontierType(), qs))) break;
+                    st = iterative ? dopt.EnactIterativeSearch(problem, src, max_ctas)
+                                   : dopt.EnactFusedSearch(problem, src, max_ctas);
+                    base = &dopt;
                     break;
                 default: std::fprintf(stderr, "unknown strategy %d\n", s); return 1;
                 }

@@ -158,6 +158,24 @@ class EnactorTwoPhase : public EnactorBase {
     }
 };
 
+// Addition (no reference counterpart): top-down expand-contract levels mixed
+// with bottom-up levels (bfs_config do_alpha / do_beta switch points).
+template <bool INSTRUMENT>
+class EnactorDirectionOptimizing : public EnactorBase {
+  public:
+    explicit EnactorDirectionOptimizing(bool debug = false) : EnactorBase(BFS_DIRECTION_OPTIMIZING, debug) {}
+    template <typename Problem>
+    bfs_status EnactFusedSearch(Problem &p, int src, int max_grid_size = 0)
+    {
+        return Enact(p, src, 1, max_grid_size);
+    }
+    template <typename Problem>
+    bfs_status EnactIterativeSearch(Problem &p, int src, int max_grid_size = 0)
+    {
+        return Enact(p, src, 0, max_grid_size);
+    }
+};
+
 template <bool INSTRUMENT>
 class EnactorHybrid : public EnactorBase {
   public:
