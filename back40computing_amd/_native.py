@@ -38,7 +38,8 @@ class BfsConfig(ctypes.Structure):
         ("tile_grab_max", ctypes.c_int32),
         ("bu_tile_blocks", ctypes.c_int32),
         ("grab_strided", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("claim_filter", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -98,6 +99,8 @@ class ServerStats(ctypes.Structure):
         ("ticks", ctypes.c_int64),
         ("max_resident", ctypes.c_int64),
         ("busy_ms", ctypes.c_double),
+        ("tick_ms", ctypes.c_double),
+        ("copy_ms", ctypes.c_double),
     ]
 
 
@@ -177,6 +180,8 @@ def load():
     _sig(lib, "bfs_server_wait", ctypes.c_int, vp, i64, i32, ctypes.POINTER(RequestInfo))
     _sig(lib, "bfs_server_get_stats", ctypes.c_int, vp, ctypes.POINTER(ServerStats))
     _sig(lib, "bfs_server_destroy", None, vp)
+    _sig(lib, "bfs_host_alloc_pinned", ctypes.c_int, i64, ctypes.POINTER(ctypes.c_void_p))
+    _sig(lib, "bfs_host_free_pinned", None, vp)
     _sig(lib, "bfs_selftest_wave_primitives", ctypes.c_int, i32, c_int32_p)
     _lib = lib
     return lib

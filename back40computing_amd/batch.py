@@ -96,11 +96,41 @@ class Batch:
             self._h = ctypes.c_void_p()
 
 
+class _PinnedBlock:
+    """Page-locked host allocation, freed when the last array over it goes."""
+
+    __slots__ = ("ptr",)
+
+    def __init__(self, nbytes):
+        self.ptr = ctypes.c_void_p()
+        _check(_native.load().bfs_host_alloc_pinned(nbytes, ctypes.byref(self.ptr)), "bfs_host_alloc_pinned")
+
+    def __del__(self):
+        try:
+            _native.load().bfs_host_free_pinned(self.ptr)
+        except Exception:
+            pass
+
+
+def pinned_int32(n):
+    """int32 numpy array of n elements in page-locked host memory."""
+    blk = _PinnedBlock(max(n, 1) * 4)
+    carr = (ctypes.c_int32 * max(n, 1)).from_address(blk.ptr.value)
+    carr._owner = blk
+    return np.ctypeslib.as_array(carr)[:n]
+
+
 class Server:
-    """Continuous-batching BFS server over a graph pool."""
+    """Continuous-batching BFS server over a graph pool.
+
+    Result buffers are page-locked host arrays from a per-graph free list:
+    the server copies each finished query's labels into one on its own
+    stream while later levels run.  wait() hands the array to the caller;
+    recycle() returns it to the free list (optional)."""
 
     def __init__(self, pool, max_slots=64, levels_per_tick=2, hbm_budget_bytes=8 << 30, direction_optimizing=True):
         self.pool = pool
+        self._free = {}              # graph id -> [array]
         cfg = _native.ServerConfig()
         cfg.max_slots = max_slots
         cfg.levels_per_tick = levels_per_tick
@@ -111,12 +141,25 @@ class Server:
                "bfs_server_create")
         self._bufs = {}
 
+    def _buffer(self, graph_id):
+        free = self._free.setdefault(graph_id, [])
+        try:
+            return free.pop()
+        except IndexError:
+            return pinned_int32(self.pool.nodes[graph_id])
+
+    def recycle(self, graph_id, buf):
+        """Returns a result array from wait() to the free list."""
+        self._free.setdefault(graph_id, []).append(buf)
+
     def submit(self, graph_id, src):
         """Returns a ticket; raises BfsError (status 8) when rejected."""
-        buf = np.empty(self.pool.nodes[graph_id], dtype=np.int32)
+        buf = self._buffer(graph_id)
         t = ctypes.c_int64()
-        _check(_native.load().bfs_server_submit(self._h, graph_id, int(src), _native.i32ptr(buf), ctypes.byref(t)),
-               "bfs_server_submit")
+        rc = _native.load().bfs_server_submit(self._h, graph_id, int(src), _native.i32ptr(buf), ctypes.byref(t))
+        if rc != 0:
+            self.recycle(graph_id, buf)
+            _check(rc, "bfs_server_submit")
         self._bufs[t.value] = buf
         return t.value
 

@@ -173,6 +173,7 @@ struct SlotTable {
         a.grab_max = cfg.tile_grab_max > 0 ? cfg.tile_grab_max : 1;
         a.bu_tile = cfg.bu_tile_blocks > 0 ? cfg.bu_tile_blocks : 1;
         a.strided = cfg.grab_strided != 2;
+        a.coherent_filter = cfg.claim_filter == 2;
         return a;
     }
 
@@ -404,6 +405,7 @@ bfs_status bfs_batch_set_config(bfs_batch *b, const bfs_config *cfg)
     if (c.tile_grab_max <= 0) c.tile_grab_max = d.tile_grab_max;
     if (c.bu_tile_blocks <= 0) c.bu_tile_blocks = d.bu_tile_blocks;
     if (c.grab_strided <= 0) c.grab_strided = d.grab_strided;
+    if (c.claim_filter <= 0) c.claim_filter = d.claim_filter;
     b->t.cfg = c;
     return BFS_OK;
 }
@@ -630,6 +632,18 @@ struct Arena {
 
 }  // namespace
 
+// A finished query whose labels are being copied out; its slot and memory
+// are reused only after the copy completes.
+struct DrainItem {
+    int slot;
+    int64_t ticket;
+    const int *d_depth;
+    size_t bytes;
+    int32_t *out;
+    int levels;
+    bfs_status status;
+};
+
 struct bfs_server {
     bfs_graph_pool *pool = nullptr;
     bfs_server_config cfg{};
@@ -643,16 +657,68 @@ struct bfs_server {
     std::deque<int64_t> pending;
     std::unordered_map<int64_t, Request> reqs;
     std::vector<int64_t> slot_ticket;
-    int resident = 0;
+    std::vector<char> slot_draining;
+    int resident = 0;                // admitted and not yet delivered
+    int running = 0;                 // admitted and not finished on the device
+    int draining = 0;                // finished, labels being copied out
     int64_t next_ticket = 1;
-    bool stop = false;
+    bool stop = false, drain_stop = false;
     bfs_status fatal = BFS_OK;
     bfs_server_stats stats{};
     std::chrono::steady_clock::time_point t0;
-    std::thread thread;
+    std::chrono::steady_clock::time_point busy_start;  // when resident last left 0
+    std::thread thread, drainer;
+    std::deque<DrainItem> drain_q;
+    std::condition_variable cv_drain;
+    hipStream_t copy_stream = nullptr;
 
     void run();
+    void drain();
 };
+
+// Result-copy thread: copies each finished query's labels to its caller's
+// buffer on a stream of its own, then delivers the result and frees the slot
+// and its memory, while the server thread keeps replaying levels.
+void bfs_server::drain()
+{
+    (void) hipSetDevice(pool->device);
+    while (true) {
+        DrainItem it;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_drain.wait(lk, [&] { return drain_stop || !drain_q.empty(); });
+            if (drain_q.empty()) return;
+            it = drain_q.front();
+            drain_q.pop_front();
+        }
+        auto c0 = std::chrono::steady_clock::now();
+        bfs_status rs = it.status;
+        if (it.out && rs == BFS_OK) {
+            hipError_t e = hipMemcpyAsync(it.out, it.d_depth, it.bytes, hipMemcpyDeviceToHost, copy_stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(copy_stream);
+            if (e != hipSuccess) rs = host_hip_fail(e, "copy result");
+        }
+        double cms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0).count();
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            Request &r = reqs[it.ticket];
+            r.levels = it.levels;
+            r.status = rs;
+            r.finish_ms = wall_ms(t0);
+            r.done = true;
+            arena.release(r.offset, r.bytes);
+            slot_ticket[it.slot] = -1;
+            slot_draining[it.slot] = 0;
+            draining--;
+            resident--;
+            stats.completed++;
+            stats.copy_ms += cms;
+            if (resident == 0) stats.busy_ms += wall_ms(busy_start);
+        }
+        cv_done.notify_all();
+        cv_work.notify_all();
+    }
+}
 
 void bfs_server::run()
 {
@@ -661,59 +727,68 @@ void bfs_server::run()
     int grid = batch_grid(pool, false, t.cfg.blocks_per_cu);
     bfs_status st = t.capture(a, grid, cfg.levels_per_tick);
     std::vector<int> admitted;
-    auto busy_start = std::chrono::steady_clock::now();
     while (true) {
         admitted.clear();
         {
             std::unique_lock<std::mutex> lk(mu);
-            if (st != BFS_OK) fatal = st;
-            cv_work.wait(lk, [&] { return stop || !pending.empty() || resident > 0 || fatal != BFS_OK; });
-            if (fatal != BFS_OK) {
-                // fail everything still queued or resident
-                for (auto &kv : reqs) {
-                    if (!kv.second.done) {
-                        kv.second.done = true;
-                        kv.second.status = fatal;
+            while (true) {
+                if (st != BFS_OK) fatal = st;
+                if (fatal != BFS_OK) {
+                    // fail everything queued or running; draining results
+                    // are delivered by the copy thread
+                    for (auto &kv : reqs) {
+                        Request &r = kv.second;
+                        bool in_drain = r.slot >= 0 && slot_draining[r.slot] && slot_ticket[r.slot] == r.ticket;
+                        if (!r.done && !in_drain) {
+                            r.done = true;
+                            r.status = fatal;
+                        }
                     }
+                    pending.clear();
+                    cv_done.notify_all();
+                    return;
                 }
-                pending.clear();
-                cv_done.notify_all();
-                return;
-            }
-            if (stop && pending.empty() && resident == 0) return;
-            // Admission: free slot and room in the HBM budget, in arrival order.
-            for (int s = 0; s < cfg.max_slots && !pending.empty(); s++) {
-                if (slot_ticket[s] >= 0) continue;
-                Request &r = reqs[pending.front()];
-                const PoolGraph &g = graphs[r.graph];
-                SlotLayout lay = slot_layout(g.nodes);
-                size_t off = 0;
-                if (!arena.alloc(lay.bytes, &off)) break;
-                r.offset = off;
-                r.bytes = lay.bytes;
-                r.slot = s;
-                r.admit_ms = wall_ms(t0);
-                slot_ticket[s] = r.ticket;
-                pending.pop_front();
-                BatchSlot &bs = t.h_slots[s];
-                memset(&bs, 0, sizeof(bs));
-                char *base = d_arena + off;
-                bs.graph = r.graph;
-                bs.src = r.src;
-                bs.depth = reinterpret_cast<int *>(base + lay.depth);
-                bs.visited = reinterpret_cast<unsigned *>(base + lay.visited);
-                bs.fbm[0] = reinterpret_cast<unsigned *>(base + lay.fbm0);
-                bs.fbm[1] = reinterpret_cast<unsigned *>(base + lay.fbm1);
-                bs.q[0] = reinterpret_cast<int *>(base + lay.q0);
-                bs.q[1] = reinterpret_cast<int *>(base + lay.q1);
-                bs.cap = (unsigned) std::max(g.nodes, 1);
-                admitted.push_back(s);
-                if (resident == 0) busy_start = std::chrono::steady_clock::now();
-                resident++;
-                stats.admitted++;
-                stats.max_resident = std::max<int64_t>(stats.max_resident, resident);
+                // Admission: free slot and room in the HBM budget, in
+                // arrival order.
+                for (int s = 0; s < cfg.max_slots && !pending.empty(); s++) {
+                    if (slot_ticket[s] >= 0) continue;
+                    Request &r = reqs[pending.front()];
+                    const PoolGraph &g = graphs[r.graph];
+                    SlotLayout lay = slot_layout(g.nodes);
+                    size_t off = 0;
+                    if (!arena.alloc(lay.bytes, &off)) break;
+                    r.offset = off;
+                    r.bytes = lay.bytes;
+                    r.slot = s;
+                    r.admit_ms = wall_ms(t0);
+                    slot_ticket[s] = r.ticket;
+                    pending.pop_front();
+                    BatchSlot &bs = t.h_slots[s];
+                    memset(&bs, 0, sizeof(bs));
+                    char *base = d_arena + off;
+                    bs.graph = r.graph;
+                    bs.src = r.src;
+                    bs.depth = reinterpret_cast<int *>(base + lay.depth);
+                    bs.visited = reinterpret_cast<unsigned *>(base + lay.visited);
+                    bs.fbm[0] = reinterpret_cast<unsigned *>(base + lay.fbm0);
+                    bs.fbm[1] = reinterpret_cast<unsigned *>(base + lay.fbm1);
+                    bs.q[0] = reinterpret_cast<int *>(base + lay.q0);
+                    bs.q[1] = reinterpret_cast<int *>(base + lay.q1);
+                    bs.cap = (unsigned) std::max(g.nodes, 1);
+                    admitted.push_back(s);
+                    if (resident == 0) busy_start = std::chrono::steady_clock::now();
+                    resident++;
+                    running++;
+                    stats.admitted++;
+                    stats.max_resident = std::max<int64_t>(stats.max_resident, resident);
+                }
+                if (running > 0) break;
+                if (stop && pending.empty() && draining == 0) return;
+                // nothing to run: wait for a submission or a freed slot
+                cv_work.wait(lk);
             }
         }
+        auto tick0 = std::chrono::steady_clock::now();
         hipError_t e = hipSuccess;
         if (!admitted.empty()) {
             int max_nodes = 1;
@@ -730,55 +805,45 @@ void bfs_server::run()
             // the seed list buffer is reused next tick: wait for the seeding
             if (e == hipSuccess) e = hipStreamSynchronize(t.stream);
         }
-        // One tick: levels_per_tick levels of every resident query.
+        // One tick: levels_per_tick levels of every running query.
         if (e == hipSuccess) e = hipGraphLaunch(t.graph_exec, t.stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(t.h_slots, t.d_slots, sizeof(BatchSlot) * cfg.max_slots, hipMemcpyDeviceToHost,
                                t.stream);
         if (e == hipSuccess) e = hipStreamSynchronize(t.stream);
+        double tms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tick0).count();
         if (e != hipSuccess) {
             st = host_hip_fail(e, "server tick");
             continue;
         }
-        // Retire finished queries.
-        std::vector<int> finished;
-        for (int s = 0; s < cfg.max_slots; s++)
-            if (slot_ticket[s] >= 0 && t.h_slots[s].active && t.h_slots[s].done) finished.push_back(s);
-        for (int s : finished) {
-            int64_t ticket;
-            Request *r;
-            {
-                std::lock_guard<std::mutex> lk(mu);
-                ticket = slot_ticket[s];
-                r = &reqs[ticket];
-            }
-            const BatchSlot &bs = t.h_slots[s];
-            bfs_status rs = bs.overflow ? BFS_ERR_OVERFLOW : BFS_OK;
-            if (r->out && rs == BFS_OK) {
-                hipError_t ce = hipMemcpy(r->out, bs.depth, sizeof(int) * (size_t) graphs[r->graph].nodes,
-                                          hipMemcpyDeviceToHost);
-                if (ce != hipSuccess) rs = host_hip_fail(ce, "copy result");
-            }
-            int levels = bs.iteration;
-            // deactivate the slot on the device
-            t.h_slots[s].active = 0;
-            (void) hipMemcpy(t.d_slots + s, t.h_slots + s, sizeof(BatchSlot), hipMemcpyHostToDevice);
-            std::lock_guard<std::mutex> lk(mu);
-            r->levels = levels;
-            r->status = rs;
-            r->finish_ms = wall_ms(t0);
-            r->done = true;
-            arena.release(r->offset, r->bytes);
-            slot_ticket[s] = -1;
-            resident--;
-            stats.completed++;
-            if (resident == 0) stats.busy_ms += wall_ms(busy_start);
-        }
+        // Hand finished queries to the copy thread.  A finished slot stays
+        // done (idle) on the device until a new query is seeded into it.
+        int handed = 0;
         {
             std::lock_guard<std::mutex> lk(mu);
             stats.ticks++;
+            stats.tick_ms += tms;
+            for (int s = 0; s < cfg.max_slots; s++) {
+                if (slot_ticket[s] < 0 || slot_draining[s]) continue;
+                const BatchSlot &bs = t.h_slots[s];
+                if (!bs.active || !bs.done) continue;
+                const Request &r = reqs[slot_ticket[s]];
+                DrainItem it;
+                it.slot = s;
+                it.ticket = slot_ticket[s];
+                it.d_depth = bs.depth;
+                it.bytes = sizeof(int) * (size_t) graphs[r.graph].nodes;
+                it.out = r.out;
+                it.levels = bs.iteration;
+                it.status = bs.overflow ? BFS_ERR_OVERFLOW : BFS_OK;
+                slot_draining[s] = 1;
+                running--;
+                draining++;
+                drain_q.push_back(it);
+                handed++;
+            }
         }
-        if (!finished.empty()) cv_done.notify_all();
+        if (handed) cv_drain.notify_one();
     }
 }
 
@@ -797,6 +862,7 @@ bfs_status bfs_server_create(bfs_graph_pool *pool, const bfs_server_config *cfg,
     s->pool = pool;
     s->cfg = c;
     auto fail_out = [&](bfs_status st) {
+        if (s->copy_stream) (void) hipStreamDestroy(s->copy_stream);
         s->t.release();
         if (s->d_arena) (void) hipFree(s->d_arena);
         if (s->d_table) (void) hipFree(s->d_table);
@@ -819,8 +885,13 @@ bfs_status bfs_server_create(bfs_graph_pool *pool, const bfs_server_config *cfg,
     if (hipMalloc(&s->d_arena, (size_t) c.hbm_budget_bytes) != hipSuccess)
         return fail_out(host_fail(BFS_ERR_OOM, "allocate the query-state budget"));
     s->arena.reset((size_t) c.hbm_budget_bytes);
+    if (hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess)
+        return fail_out(host_fail(BFS_ERR_HIP, "create the result-copy stream"));
     s->slot_ticket.assign(c.max_slots, -1);
+    s->slot_draining.assign(c.max_slots, 0);
     s->t0 = std::chrono::steady_clock::now();
+    s->busy_start = s->t0;
+    s->drainer = std::thread([s] { s->drain(); });
     s->thread = std::thread([s] { s->run(); });
     *out = s;
     return BFS_OK;
@@ -885,6 +956,20 @@ bfs_status bfs_server_get_stats(bfs_server *s, bfs_server_stats *stats)
     return BFS_OK;
 }
 
+bfs_status bfs_host_alloc_pinned(int64_t bytes, void **ptr)
+{
+    if (!ptr || bytes <= 0) return host_fail(BFS_ERR_INVALID_ARG, "invalid pinned allocation");
+    *ptr = nullptr;
+    if (hipHostMalloc(ptr, (size_t) bytes, hipHostMallocDefault) != hipSuccess)
+        return host_fail(BFS_ERR_OOM, "pinned host allocation");
+    return BFS_OK;
+}
+
+void bfs_host_free_pinned(void *ptr)
+{
+    if (ptr) (void) hipHostFree(ptr);
+}
+
 void bfs_server_destroy(bfs_server *s)
 {
     if (!s) return;
@@ -894,6 +979,15 @@ void bfs_server_destroy(bfs_server *s)
     }
     s->cv_work.notify_all();
     if (s->thread.joinable()) s->thread.join();
+    // the server thread returns once every result is delivered (or on a
+    // fatal error); stop the copy thread after whatever it still holds
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->drain_stop = true;
+    }
+    s->cv_drain.notify_all();
+    if (s->drainer.joinable()) s->drainer.join();
+    if (s->copy_stream) (void) hipStreamDestroy(s->copy_stream);
     s->t.release();
     if (s->d_arena) (void) hipFree(s->d_arena);
     if (s->d_table) (void) hipFree(s->d_table);

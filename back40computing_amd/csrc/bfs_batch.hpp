@@ -66,6 +66,7 @@ struct BatchArgs {
     int grab_max;             // most tiles taken per tile-counter atomic
     int bu_tile;              // kBlock-vertex blocks per bottom-up tile
     int strided;              // grab layout: tiles gridDim apart, else consecutive
+    int coherent_filter;      // claim pre-test reads the device-coherent visited word
 };
 
 }  // namespace b40c_hip

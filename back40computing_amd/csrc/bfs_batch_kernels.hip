@@ -195,9 +195,10 @@ struct TdEdge {
     const BatchGraph &g;
     int next_depth;
     SlotOut &o;
+    bool coherent;
     __device__ __forceinline__ void operator()(bool ok, int u, int par) const
     {
-        bool fresh = ok && claim_exact(sl.visited, u);
+        bool fresh = ok && claim_exact(sl.visited, u, coherent);
         if (fresh) {
             sl.depth[u] = next_depth;
             o.fe += (unsigned) (g.row[u + 1] - g.row[u]);
@@ -222,7 +223,7 @@ __device__ void batch_td_tile(const BatchArgs &a, BatchLds &L, int q, unsigned l
         beg = g.row[v];
         end = g.row[v + 1];
     }
-    gather_rows<false>(beg, end, v, g.col, a.wave_t, a.cta_t, hub, L.base, TdEdge{sl, g, L.it[q] + 1, o});
+    gather_rows<false>(beg, end, v, g.col, a.wave_t, a.cta_t, hub, L.base, TdEdge{sl, g, L.it[q] + 1, o, a.coherent_filter != 0});
 }
 
 // Grid-tier phase of a level: every workgroup takes kHubChunk-edge chunks of
@@ -261,7 +262,7 @@ __device__ void batch_hub_phase(const BatchArgs &a, BatchLds &L, int level)
         out_open(o, a, L, q);
         BatchSlot &sl = a.slots[q];
         const BatchGraph g = a.graphs[sl.graph];
-        TdEdge edge{sl, g, L.it[q] + 1, o};
+        TdEdge edge{sl, g, L.it[q] + 1, o, a.coherent_filter != 0};
         for (int k = b; k < e; k += kBlock) {
             int idx = k + threadIdx.x;
             bool ok = idx < e;

@@ -61,7 +61,8 @@ const bfs_config kDefaults = {
     /*tile_grab_max*/ 32,
     /*bu_tile_blocks*/ 1,
     /*grab_strided*/ 1,
-    {0, 0, 0}};
+    /*claim_filter*/ 1,
+    {0, 0}};
 
 constexpr unsigned kHubCapacity = 1u << 16;  // grid-tier entries per step list
 
@@ -145,6 +146,7 @@ SearchArgs search_args(const bfs_problem *p, int grid)
     s.cta_t = p->cfg.cta_gather_threshold;
     s.grid_t = p->cfg.grid_gather_threshold > 0 ? p->cfg.grid_gather_threshold : 0x7fffffff;
     s.grab_max = p->cfg.tile_grab_max > 0 ? p->cfg.tile_grab_max : 1;
+    s.coherent_filter = p->cfg.claim_filter == 2;
     s.fast_t = p->cfg.fast_path_threshold;
     s.sat_t = p->cfg.saturation_threshold > 0 ? p->cfg.saturation_threshold
               : (p->cfg.saturation_threshold < 0 ? -1 : grid * 256);
@@ -510,6 +512,7 @@ bfs_status bfs_problem_set_config(bfs_problem *p, const bfs_config *cfg)
     if (c.blocks_per_cu == 0) c.blocks_per_cu = kDefaults.blocks_per_cu;
     if (c.grid_gather_threshold == 0) c.grid_gather_threshold = kDefaults.grid_gather_threshold;
     if (c.grid_gather_threshold < 0) c.grid_gather_threshold = -1;  // grid tier off
+    if (c.claim_filter == 0) c.claim_filter = kDefaults.claim_filter;
     if (c.do_alpha == 0) c.do_alpha = kDefaults.do_alpha;
     if (c.do_beta == 0) c.do_beta = kDefaults.do_beta;
     if (c.do_alpha < 1 || c.do_beta < 1) return fail(BFS_ERR_INVALID_ARG, "invalid direction switch points");

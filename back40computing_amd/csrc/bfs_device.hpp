@@ -143,11 +143,16 @@ __device__ __forceinline__ unsigned add_agent(unsigned *p, unsigned v)
 // label check; racing discoverers may both succeed, which only duplicates
 // work because they write the same depth.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool claim_exact(unsigned *visited, int u)
+// The pre-test read decides only whether to try the atomic.  The cached read
+// may miss bits other workgroups set during this level (device-scope atomics
+// are performed beyond this XCD's L2, which keeps its old copy); the
+// coherent read sees them at the cost of a longer trip.
+__device__ __forceinline__ bool claim_exact(unsigned *visited, int u, bool coherent = false)
 {
     unsigned *w = visited + (u >> 5);
     unsigned bit = 1u << (u & 31);
-    if (*w & bit) return false;
+    unsigned cur = coherent ? __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *w;
+    if (cur & bit) return false;
     unsigned old = __hip_atomic_fetch_or(w, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return (old & bit) == 0;
 }
@@ -163,9 +168,9 @@ __device__ __forceinline__ bool claim_best_effort(unsigned *visited, const int *
 }
 
 template <bool EXACT>
-__device__ __forceinline__ bool claim(unsigned *visited, const int *depth, int u)
+__device__ __forceinline__ bool claim(unsigned *visited, const int *depth, int u, bool coherent = false)
 {
-    if (EXACT) return claim_exact(visited, u);
+    if (EXACT) return claim_exact(visited, u, coherent);
     return claim_best_effort(visited, depth, u);
 }
 

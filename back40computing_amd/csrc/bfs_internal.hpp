@@ -47,6 +47,7 @@ struct SearchArgs {
     int wave_t, cta_t, grid_t, fast_t, sat_t;
     int instrument;
     int grab_max;          // most tiles taken per tile-counter atomic
+    int coherent_filter;   // claim pre-test reads the device-coherent visited word
 };
 
 // Loop position: step index (selects queues and counters), BFS iteration

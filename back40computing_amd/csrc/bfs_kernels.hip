@@ -219,7 +219,7 @@ __device__ void body_expand_contract(const GraphArgs &g, const SearchArgs &s, Cu
     bool labeled = false;
     unsigned fe = 0;
     auto edge = [&](bool ok, int u, int par) {
-        bool fresh = ok && claim<EXACT>(s.visited, s.depth, u);
+        bool fresh = ok && claim<EXACT>(s.visited, s.depth, u, s.coherent_filter);
         if (fresh) {
             s.depth[u] = next_depth;
             if (PRED) s.pred[u] = par;
@@ -346,7 +346,7 @@ __device__ void body_contract_expand(const GraphArgs &g, const SearchArgs &s, Cu
         tile_phase<PRED>(g, s, c, n, hub_list(s, c.step, hubs), lds, [&](unsigned i, int &beg, int &end, int &par) {
             int u = in[i];
             int p = PRED ? pin[i] : -1;
-            if (history_pass<PRED>(lds, u) && claim<EXACT>(s.visited, s.depth, u)) {
+            if (history_pass<PRED>(lds, u) && claim<EXACT>(s.visited, s.depth, u, s.coherent_filter)) {
                 s.depth[u] = c.iteration;
                 if (PRED) s.pred[u] = p;
                 beg = g.row[u];
@@ -400,7 +400,7 @@ __device__ void body_contract(const GraphArgs &g, const SearchArgs &s, Cursor c,
             u = in[i];
             if (PRED) p = pin[i];
         }
-        bool keep = u >= 0 && history_pass<PRED>(lds, u) && claim<EXACT>(s.visited, s.depth, u);
+        bool keep = u >= 0 && history_pass<PRED>(lds, u) && claim<EXACT>(s.visited, s.depth, u, s.coherent_filter);
         if (keep) {
             s.depth[u] = c.iteration;
             if (PRED) s.pred[u] = p;

@@ -159,8 +159,9 @@ def bench_serving(b40c, np, rmat_scale, rates, queries_per_rate):
                 pass
             tickets.append(srv.submit(gi, int(src_tables[gi][si])))
         lat, finish, first_submit = [], [], None
-        for tk in tickets:
-            _, info = srv.wait(tk)
+        for (gi, _), tk in zip(picks, tickets):
+            d, info = srv.wait(tk)
+            srv.recycle(gi, d)
             lat.append(info["finish_ms"] - info["submit_ms"])
             finish.append(info["finish_ms"])
             first_submit = info["submit_ms"] if first_submit is None else min(first_submit, info["submit_ms"])
@@ -170,7 +171,9 @@ def bench_serving(b40c, np, rmat_scale, rates, queries_per_rate):
         results.append({"offered_rate_per_s": rate, "queries": queries_per_rate,
                         "queries_per_s": round(queries_per_rate / span, 1),
                         "latency_p50_ms": pct(lat, 50), "latency_p99_ms": pct(lat, 99),
-                        "max_resident": st["max_resident"], "rejected": st["rejected"], "ticks": st["ticks"]})
+                        "max_resident": st["max_resident"], "rejected": st["rejected"], "ticks": st["ticks"],
+                        "tick_ms": round(st["tick_ms"], 1), "copy_ms": round(st["copy_ms"], 1),
+                        "span_ms": round(span * 1e3, 1)})
         log("serving rate %g/s: %.1f q/s, p50 %.3f ms, p99 %.3f ms" % (
             rate, results[-1]["queries_per_s"], results[-1]["latency_p50_ms"], results[-1]["latency_p99_ms"]))
     pool.destroy()
@@ -195,7 +198,7 @@ def main():
     ap.add_argument("--batched-reps", type=int, default=5)
     ap.add_argument("--serving", action="store_true", help="add the serving sweep")
     ap.add_argument("--serving-rmat-scale", type=int, default=24)
-    ap.add_argument("--serving-rates", default="1000,4000,16000,64000")
+    ap.add_argument("--serving-rates", default="100,200,400,800,1600,3200")
     ap.add_argument("--serving-queries", type=int, default=1000)
     ap.add_argument("--calibrate-bytes", type=int, default=0,
                     help="also run the plain copy kernel over this many bytes (counter calibration)")

@@ -112,7 +112,10 @@ typedef struct bfs_config {
     int32_t bu_tile_blocks;         /* batched: 256-vertex blocks per bottom-up tile */
     int32_t grab_strided;           /* batched: 1 = a grab's tiles are gridDim apart
                                        (narrow in-flight window), 2 = consecutive */
-    int32_t reserved[3];
+    int32_t claim_filter;           /* exact claims: the visited-word test before the
+                                       atomic reads 1 = the cached copy, 2 = the
+                                       device-coherent value */
+    int32_t reserved[2];
 } bfs_config;
 
 typedef struct bfs_stats {

@@ -100,6 +100,8 @@ typedef struct bfs_server_stats {
     int64_t ticks;            /* level-graph replays */
     int64_t max_resident;     /* most queries resident at once */
     double busy_ms;           /* host time with at least one query resident */
+    double tick_ms;           /* host time in level-graph replays (GPU busy) */
+    double copy_ms;           /* result-copy thread time (overlaps the ticks) */
 } bfs_server_stats;
 
 bfs_status bfs_server_create(bfs_graph_pool *pool, const bfs_server_config *cfg, bfs_server **out);
@@ -111,8 +113,13 @@ bfs_status bfs_server_submit(bfs_server *s, int32_t graph_id, int32_t src, int32
 /* Waits for a ticket (timeout_ms < 0: forever).  BFS_ERR_TIMEOUT on timeout. */
 bfs_status bfs_server_wait(bfs_server *s, int64_t ticket, int32_t timeout_ms, bfs_request_info *info);
 bfs_status bfs_server_get_stats(bfs_server *s, bfs_server_stats *stats);
-/* Stops admitting, finishes resident queries and joins the server thread. */
+/* Stops admitting, finishes resident queries and joins the server threads. */
 void bfs_server_destroy(bfs_server *s);
+
+/* Page-locked host memory for result buffers: copies into it run at full
+ * link speed and overlap the next levels.  Any host memory works. */
+bfs_status bfs_host_alloc_pinned(int64_t bytes, void **ptr);
+void bfs_host_free_pinned(void *ptr);
 
 #ifdef __cplusplus
 }
