@@ -215,7 +215,7 @@ __device__ void batch_td_tile(const BatchArgs &a, BatchLds &L, int q, unsigned l
     const BatchGraph g = a.graphs[sl.graph];
     const int step = L.step[q];
     const unsigned n = (unsigned) L.n[q];
-    HubList hub{a.hubs[level & 1], a.hub_cap, &a.ctl->hub_alloc[level & 1], a.grid_t, q};
+    HubList hub{a.hubs[level & 1], a.hub_cap, &a.ctl->hub_alloc[level & 1], a.grid_t, q, kHubChunk};
     unsigned i = local * kBlock + threadIdx.x;
     int v = -1, beg = 0, end = 0;
     if (i < n) {

@@ -61,7 +61,8 @@ struct alignas(256) Control {
 };
 
 // Grid-tier work item: a row too long for one workgroup, split into
-// kHubChunk-edge chunks [first_chunk, first_chunk + ceil(len / kHubChunk)).
+// chunk-edge chunks [first_chunk, first_chunk + ceil(len / chunk)); the
+// chunk is kHubChunk, or kBlock for steps with fewer tiles than workgroups.
 struct HubEntry {
     int beg, end, par;
     unsigned first_chunk;
@@ -275,6 +276,7 @@ struct HubList {
     unsigned long long *alloc;
     int grid_t;        // rows at least this long go to the grid tier
     int tag;           // stored with every published row
+    int chunk;         // edges per chunk
 };
 
 template <bool PRED, typename Fn>
@@ -290,7 +292,7 @@ __device__ __forceinline__ void gather_rows(int beg, int end, int par, const int
     // (processed after this tile phase).  A full list falls back to the CTA
     // tier below.
     if (len >= hub.grid_t) {
-        unsigned long long nch = (unsigned) ((len + kHubChunk - 1) / kHubChunk);
+        unsigned long long nch = (unsigned) ((len + hub.chunk - 1) / hub.chunk);
         unsigned long long old = __hip_atomic_fetch_add(hub.alloc, (1ull << 32) | nch, __ATOMIC_RELAXED,
                                                         __HIP_MEMORY_SCOPE_AGENT);
         unsigned slot = (unsigned) (old >> 32);

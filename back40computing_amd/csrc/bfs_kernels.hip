@@ -103,14 +103,25 @@ __device__ __forceinline__ WaveQueue<PRED> make_queue(const SearchArgs &s, int s
     return wq;
 }
 
-__device__ __forceinline__ HubList hub_list(const SearchArgs &s, int step, bool enabled)
+// Steps with fewer tiles than the grid has workgroups (a level of a handful
+// of hub vertices): most workgroups would idle while a few stride long rows
+// alone, so every row the workgroup tier would take goes to the grid tier
+// instead, in kBlock-edge chunks.  Grid-uniform (n is read after a barrier).
+__device__ __forceinline__ bool narrow_step(unsigned n) { return (n + kBlock - 1) / kBlock < gridDim.x; }
+
+__device__ __forceinline__ int hub_chunk(unsigned n) { return narrow_step(n) ? kBlock : kHubChunk; }
+
+// Grid-tier list of a gathering step over n input items.
+__device__ __forceinline__ HubList hub_list(const SearchArgs &s, int step, bool enabled, unsigned n)
 {
     HubList h;
     h.entries = s.hubs[step & 1];
     h.cap = s.hub_cap;
     h.alloc = &s.ctl->hub_alloc[step & 1];
     h.grid_t = enabled ? s.grid_t : 0x7fffffff;
+    if (enabled && narrow_step(n)) h.grid_t = min(h.grid_t, s.cta_t);
     h.tag = 0;
+    h.chunk = hub_chunk(n);
     return h;
 }
 
@@ -162,11 +173,11 @@ __device__ __forceinline__ void tile_phase(const GraphArgs &g, const SearchArgs 
     }
 }
 
-// Hub phase: every workgroup takes kHubChunk-edge chunks of the rows that
+// Hub phase: every workgroup takes hub_chunk(n)-edge chunks of the rows that
 // the tile phase published, until all chunks are done.  Must follow a grid
 // barrier (or kernel boundary) after the tile phase.
 template <bool PRED, typename Edge>
-__device__ __forceinline__ void hub_phase(const GraphArgs &g, const SearchArgs &s, Cursor c, Lds<PRED> &lds,
+__device__ __forceinline__ void hub_phase(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds,
                                           Edge &&edge)
 {
     unsigned long long a = __hip_atomic_load(&s.ctl->hub_alloc[c.step & 1], __ATOMIC_RELAXED,
@@ -176,7 +187,8 @@ __device__ __forceinline__ void hub_phase(const GraphArgs &g, const SearchArgs &
     if (entries == 0) return;
     const HubEntry *list = s.hubs[c.step & 1];
     HubEntry lastent = list[entries - 1];
-    unsigned total = lastent.first_chunk + (unsigned) ((lastent.end - lastent.beg + kHubChunk - 1) / kHubChunk);
+    const int chunk = hub_chunk(n);
+    unsigned total = lastent.first_chunk + (unsigned) ((lastent.end - lastent.beg + chunk - 1) / chunk);
     while (true) {
         unsigned ch = next_item(&s.ctl->hub_steal[c.step & 1], lds);
         if (ch >= total) break;
@@ -189,9 +201,9 @@ __device__ __forceinline__ void hub_phase(const GraphArgs &g, const SearchArgs &
                 else hi = mid - 1;
             }
             HubEntry e = list[lo];
-            int b = e.beg + (int) (ch - e.first_chunk) * kHubChunk;
+            int b = e.beg + (int) (ch - e.first_chunk) * chunk;
             lds.cta_beg = b;
-            lds.cta_end = min(e.end, b + kHubChunk);
+            lds.cta_end = min(e.end, b + chunk);
             lds.cta_par = e.par;
         }
         __syncthreads();
@@ -229,14 +241,14 @@ __device__ void body_expand_contract(const GraphArgs &g, const SearchArgs &s, Cu
         wq.push(fresh, u, par);
     };
     if (sub == 0) {
-        tile_phase<PRED>(g, s, c, n, hub_list(s, c.step, hubs), lds, [&](unsigned i, int &beg, int &end, int &par) {
+        tile_phase<PRED>(g, s, c, n, hub_list(s, c.step, hubs, n), lds, [&](unsigned i, int &beg, int &end, int &par) {
             int v = in[i];
             beg = g.row[v];
             end = g.row[v + 1];
             par = v;
         }, edge);
     } else {
-        hub_phase<PRED>(g, s, c, lds, edge);
+        hub_phase<PRED>(g, s, c, n, lds, edge);
     }
     wq.flush();
     note_depth(s.ctl, labeled, next_depth);
@@ -343,7 +355,7 @@ __device__ void body_contract_expand(const GraphArgs &g, const SearchArgs &s, Cu
     bool labeled = false;
     auto edge = [&](bool ok, int w, int par) { wq.push(ok, w, par); };
     if (sub == 0) {
-        tile_phase<PRED>(g, s, c, n, hub_list(s, c.step, hubs), lds, [&](unsigned i, int &beg, int &end, int &par) {
+        tile_phase<PRED>(g, s, c, n, hub_list(s, c.step, hubs, n), lds, [&](unsigned i, int &beg, int &end, int &par) {
             int u = in[i];
             int p = PRED ? pin[i] : -1;
             if (history_pass<PRED>(lds, u) && claim<EXACT>(s.visited, s.depth, u, s.coherent_filter)) {
@@ -356,7 +368,7 @@ __device__ void body_contract_expand(const GraphArgs &g, const SearchArgs &s, Cu
             }
         }, edge);
     } else {
-        hub_phase<PRED>(g, s, c, lds, edge);
+        hub_phase<PRED>(g, s, c, n, lds, edge);
     }
     wq.flush();
     note_depth(s.ctl, labeled, c.iteration);
@@ -370,14 +382,14 @@ __device__ void body_expand(const GraphArgs &g, const SearchArgs &s, Cursor c, u
     const int *in = s.q[c.step & 1];
     auto edge = [&](bool ok, int w, int par) { wq.push(ok, w, par); };
     if (sub == 0) {
-        tile_phase<PRED>(g, s, c, n, hub_list(s, c.step, hubs), lds, [&](unsigned i, int &beg, int &end, int &par) {
+        tile_phase<PRED>(g, s, c, n, hub_list(s, c.step, hubs, n), lds, [&](unsigned i, int &beg, int &end, int &par) {
             int v = in[i];
             beg = g.row[v];
             end = g.row[v + 1];
             par = v;
         }, edge);
     } else {
-        hub_phase<PRED>(g, s, c, lds, edge);
+        hub_phase<PRED>(g, s, c, n, lds, edge);
     }
     wq.flush();
 }
@@ -665,7 +677,8 @@ __global__ __launch_bounds__(kBlock) void hub_kernel(GraphArgs g, SearchArgs s, 
     __shared__ Lds<PRED> lds;
     if (!hubs_pending(s, c.step)) return;
     init_lds(lds);
-    run_body<PRED, EXACT>(PHASE, g, s, c, 0, lds, 1, true);
+    // the tile step's input length: it sets the chunk size of the list
+    run_body<PRED, EXACT>(PHASE, g, s, c, frontier_length(s, c.step), lds, 1, true);
 }
 
 __global__ __launch_bounds__(kBlock) void frontier_bitmap_kernel(GraphArgs g, SearchArgs s, Cursor c, int pass)
