@@ -42,25 +42,6 @@ __device__ __forceinline__ unsigned long long ld64(const unsigned long long *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Exclusive prefix sum over the workgroup (one value per thread): wave DPP
-// sums, then the four wave totals through LDS.
-__device__ __forceinline__ int block_exclusive_sum(int x, int *wave_tot, int *total)
-{
-    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    int incl = wave_inclusive_sum(x);
-    if (lane == kWave - 1) wave_tot[wave] = incl;
-    __syncthreads();
-    int off = 0, tot = 0;
-    for (int w = 0; w < kWaves; w++) {
-        int t = wave_tot[w];
-        if (w < wave) off += t;
-        tot += t;
-    }
-    *total = tot;
-    __syncthreads();
-    return off + incl - x;
-}
-
 // Direction rule of the direction-optimizing search, per query.
 __device__ __forceinline__ int batch_direction(const BatchArgs &a, const BatchSlot &sl, const BatchGraph &g,
                                                int phase, int step, unsigned n)

@@ -274,6 +274,8 @@ bfs_status enact_direction_iterative(bfs_problem *p, const GraphArgs &g, const S
         }
         if (dir == kPhaseBottomUp && c.phase != kPhaseBottomUp)
             HIP_TRY(launch_build_frontier_bitmap(s, c, grid, p->stream, launches), "build frontier bitmap");
+        else if (dir != kPhaseBottomUp && c.phase == kPhaseBottomUp)
+            HIP_TRY(launch_queue_from_bitmap(s, c, grid, p->stream, launches), "queue from bitmap");
         c.phase = dir;
         HIP_TRY(launch_step(c.phase, p->mark_pred, exact, g, s, c, grid, p->stream, launches), "launch step");
         c.step++;

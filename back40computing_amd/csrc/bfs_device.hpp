@@ -41,7 +41,7 @@ struct alignas(256) Control {
     int fast_iteration;
     int fast_phase;
     int max_depth;            // deepest label written
-    int pad0[1];
+    unsigned compact;         // bitmap -> queue reservations (zeroed every step)
     unsigned long long total_queued;
     unsigned long long busy_cycles;   // instrumented: sum over workgroups of step time
     unsigned long long life_cycles;   // instrumented: sum over workgroups of kernel time
@@ -104,6 +104,25 @@ __device__ __forceinline__ int wave_inclusive_sum(int x)
     x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);  // row_bcast:15 -> rows 1, 3
     x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);  // row_bcast:31 -> rows 2, 3
     return x;
+}
+
+// Exclusive prefix sum over the workgroup (one value per thread): wave DPP
+// sums, then the four wave totals through LDS.
+__device__ __forceinline__ int block_exclusive_sum(int x, int *wave_tot, int *total)
+{
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    int incl = wave_inclusive_sum(x);
+    if (lane == kWave - 1) wave_tot[wave] = incl;
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int w = 0; w < kWaves; w++) {
+        int t = wave_tot[w];
+        if (w < wave) off += t;
+        tot += t;
+    }
+    *total = tot;
+    __syncthreads();
+    return off + incl - x;
 }
 
 // ---------------------------------------------------------------------------
@@ -199,6 +218,13 @@ struct Lds {
     unsigned tile;
     unsigned nb;                         // broadcast frontier length
     int step, iteration;
+    // end-of-step reduction over the waves (one global update per workgroup)
+    unsigned ep_count[kWaves];
+    unsigned ep_extra[kWaves];
+    unsigned long long ep_fe[kWaves];
+    int ep_lab[kWaves];
+    int ep_scan[kWaves];
+    unsigned ep_base;
 };
 
 // ---------------------------------------------------------------------------
@@ -289,16 +315,32 @@ __device__ __forceinline__ void gather_rows(int beg, int end, int par, const int
     int len = end - beg;
 
     // Grid tier: publish the row as chunks for every workgroup of the grid
-    // (processed after this tile phase).  A full list falls back to the CTA
-    // tier below.
-    if (len >= hub.grid_t) {
-        unsigned long long nch = (unsigned) ((len + hub.chunk - 1) / hub.chunk);
-        unsigned long long old = __hip_atomic_fetch_add(hub.alloc, (1ull << 32) | nch, __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_AGENT);
-        unsigned slot = (unsigned) (old >> 32);
-        if (slot < hub.cap) {
-            hub.entries[slot] = HubEntry{beg, end, par, (unsigned) old, hub.tag};
-            len = 0;
+    // (processed after this tile phase).  The wave reserves list entries and
+    // chunk numbers for all its long rows with one atomic; entries keep
+    // lane order, so the list stays sorted by first chunk.  A full list
+    // falls back to the CTA tier below.
+    {
+        const bool pub = len >= hub.grid_t;
+        const uint64_t pm = ballot(pub);
+        if (pm) {
+            const int nch = pub ? (len + hub.chunk - 1) / hub.chunk : 0;
+            const int incl = wave_inclusive_sum(nch);
+            const unsigned total = (unsigned) readlane(incl, kWave - 1);
+            unsigned lo = 0, hi = 0;
+            if (lane == 0) {
+                unsigned long long old = __hip_atomic_fetch_add(
+                    hub.alloc, ((unsigned long long) __popcll(pm) << 32) | total, __ATOMIC_RELAXED,
+                    __HIP_MEMORY_SCOPE_AGENT);
+                lo = (unsigned) old;
+                hi = (unsigned) (old >> 32);
+            }
+            lo = (unsigned) uniform((int) lo);
+            hi = (unsigned) uniform((int) hi);
+            const unsigned slot = hi + (unsigned) lanes_below(pm);
+            if (pub && slot < hub.cap) {
+                hub.entries[slot] = HubEntry{beg, end, par, lo + (unsigned) (incl - nch), hub.tag};
+                len = 0;
+            }
         }
     }
 

@@ -65,6 +65,8 @@ enum LoopKind : int { kLoopEC = 0, kLoopCE = 1, kLoopTP = 2, kLoopDO = 3 };
 hipError_t launch_init_search(const SearchArgs &s, const int *row, int src, int label_source, hipStream_t st);
 // Sets the frontier bitmap fbm[iteration & 1] from the queue of step c.
 hipError_t launch_build_frontier_bitmap(const SearchArgs &s, Cursor c, int grid, hipStream_t st, int *launches);
+// Bitmap fbm[c.iteration & 1] -> input queue of step c (bottom-up -> top-down).
+hipError_t launch_queue_from_bitmap(const SearchArgs &s, Cursor c, int grid, hipStream_t st, int *launches);
 hipError_t launch_reset_labels(int *depth, int *pred, unsigned *visited, int nodes, hipStream_t st);
 // One step as a kernel launch; gathering phases are followed by the
 // grid-tier launch for rows published to the hub list.

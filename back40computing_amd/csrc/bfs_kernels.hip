@@ -125,25 +125,59 @@ __device__ __forceinline__ HubList hub_list(const SearchArgs &s, int step, bool 
     return h;
 }
 
-// Records the deepest label written by this wave in this step.
-__device__ __forceinline__ void note_depth(Control *ctl, bool labeled, int d)
+// End of a step for the whole workgroup (every thread calls it): the waves'
+// staged queue entries are reserved with one atomic, and their degree sums
+// and depth note are folded into one update each.  Per-wave updates would
+// put four times as many device-scope atomics on the same few addresses,
+// where they serialise across the grid.
+//
+// `uncounted` (per lane) adds frontier members that are not staged in the
+// queue: a bottom-up level records its output only in the frontier bitmap,
+// and the queue length alone carries its size.
+template <bool PRED>
+__device__ void finish_step(WaveQueue<PRED> &wq, Lds<PRED> &lds, Control *ctl, int step, bool labeled, int depth,
+                            unsigned fe, bool track, int uncounted = 0)
 {
-    if (ballot(labeled) != 0 && lane_id() == 0) atomicMax(&ctl->max_depth, d);
-}
-
-// Adds the wave's per-lane degree sums of newly labelled vertices to the
-// next frontier's edge count and to the visited-edge total (64-bit, from two
-// 16-bit halves so the DPP sums cannot overflow).
-__device__ __forceinline__ void note_edges(Control *ctl, int step, unsigned fe)
-{
+    const int lane = lane_id(), wave = threadIdx.x / kWave;
     int lo = wave_inclusive_sum((int) (fe & 0xFFFFu));
     int hi = wave_inclusive_sum((int) (fe >> 16));
-    unsigned long long tot = ((unsigned long long) (unsigned) readlane(hi, kWave - 1) << 16) +
-                             (unsigned long long) (unsigned) readlane(lo, kWave - 1);
-    if (tot != 0 && lane_id() == 0) {
-        __hip_atomic_fetch_add(&ctl->fedges[(step + 1) % 3], tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(&ctl->visited_edges, tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int extra = wave_inclusive_sum(uncounted);
+    bool any = ballot(labeled) != 0;
+    if (lane == kWave - 1) {  // the inclusive sums end in the last lane
+        lds.ep_count[wave] = (unsigned) wq.count;
+        lds.ep_extra[wave] = (unsigned) extra;
+        lds.ep_fe[wave] = ((unsigned long long) (unsigned) hi << 16) + (unsigned long long) (unsigned) lo;
+        lds.ep_lab[wave] = any ? 1 : 0;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned tot = 0, ext = 0;
+        unsigned long long fsum = 0;
+        int lab = 0;
+        for (int w = 0; w < kWaves; w++) {
+            tot += lds.ep_count[w];
+            ext += lds.ep_extra[w];
+            fsum += lds.ep_fe[w];
+            lab |= lds.ep_lab[w];
+        }
+        lds.ep_base = (tot | ext) ? add_agent(wq.out_len, tot + ext) : 0u;
+        if (track && fsum) {
+            __hip_atomic_fetch_add(&ctl->fedges[(step + 1) % 3], fsum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&ctl->visited_edges, fsum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lab) atomicMax(&ctl->max_depth, depth);
+    }
+    __syncthreads();
+    unsigned off = lds.ep_base;
+    for (int w = 0; w < wave; w++) off += lds.ep_count[w];
+    const unsigned cnt = (unsigned) wq.count;
+    const unsigned lim = off < wq.cap ? min(cnt, wq.cap - off) : 0u;
+    if (lim < cnt && lane == 0) st_agent(wq.overflow, 1u);
+    for (unsigned i = (unsigned) lane; i < lim; i += kWave) {
+        wq.out[off + i] = wq.buf[i];
+        if (PRED) wq.pout[off + i] = wq.pbuf[i];
+    }
+    wq.count = 0;
 }
 
 __device__ __forceinline__ unsigned long long ld64_agent(const unsigned long long *p)
@@ -250,9 +284,7 @@ __device__ void body_expand_contract(const GraphArgs &g, const SearchArgs &s, Cu
     } else {
         hub_phase<PRED>(g, s, c, n, lds, edge);
     }
-    wq.flush();
-    note_depth(s.ctl, labeled, next_depth);
-    if (s.track_edges) note_edges(s.ctl, c.step, fe);
+    finish_step<PRED>(wq, lds, s.ctl, c.step, labeled, next_depth, fe, s.track_edges != 0);
 }
 
 // Bottom-up step of a direction-optimizing search (Beamer, Asanovic,
@@ -277,11 +309,19 @@ __device__ void body_bottom_up(const GraphArgs &g, const SearchArgs &s, Cursor c
     auto in_frontier = [&](int u) { return ((cur[u >> 5] >> (u & 31)) & 1u) != 0; };
     bool labeled = false;
     unsigned fe = 0;
-    TileGrab tg = tile_grab(s, (unsigned) g.nodes);
-    while (true) {
-        unsigned base = next_tile(&s.ctl->steal[c.step & 1], lds, tg) * kBlock;
-        if (base >= (unsigned) g.nodes) break;
-        int v = (int) (base + threadIdx.x);
+    int found = 0;
+    // Static partition: every wave owns one contiguous run of 64-vertex
+    // groups (two whole bitmap words each) and walks it with no tile
+    // counter and no workgroup barrier, so the waves' dependent load chains
+    // overlap freely.  Every vertex costs about the same here, and vertex
+    // ids are scrambled, so the runs balance without stealing.
+    const unsigned groups = ((unsigned) g.nodes + kWave - 1) / kWave;
+    const unsigned nwaves = gridDim.x * kWaves;
+    const unsigned gw = blockIdx.x * kWaves + threadIdx.x / kWave;
+    const unsigned per = (groups + nwaves - 1) / nwaves;
+    const unsigned g_end = min(groups, (gw + 1) * per);
+    for (unsigned gi = gw * per; gi < g_end; gi++) {
+        int v = (int) (gi * kWave + (unsigned) lane);
         unsigned w = (unsigned) v >> 5;
         bool valid = v < g.nodes;
         unsigned vis = valid ? s.visited[w] : ~0u;
@@ -338,11 +378,40 @@ __device__ void body_bottom_up(const GraphArgs &g, const SearchArgs &s, Cursor c
             if (bits) s.visited[w] = vis | bits;
             nxt[w] = bits;
         }
-        wq.push(got, v, parent);
+        found += got ? 1 : 0;
     }
-    wq.flush();
-    note_depth(s.ctl, labeled, next_depth);
-    note_edges(s.ctl, c.step, fe);
+    // The level's output is the bitmap nxt; only its size goes to the queue
+    // length (queue_from_bitmap materialises the queue if the search turns
+    // top-down again).
+    finish_step<PRED>(wq, lds, s.ctl, c.step, labeled, next_depth, fe, true, found);
+}
+
+// Writes the members of frontier bitmap fbm[c.iteration & 1] into the input
+// queue of step c (a bottom-up level records its output only as a bitmap).
+// Grid-wide; a grid barrier or kernel boundary must follow before the queue
+// is read.  Reservations come from ctl->compact, which the previous step
+// zeroed.
+template <bool PRED>
+__device__ void queue_from_bitmap(const GraphArgs &g, const SearchArgs &s, Cursor c, Lds<PRED> &lds)
+{
+    const unsigned *bm = s.fbm[c.iteration & 1];
+    int *out = s.q[c.step & 1];
+    const unsigned words = ((unsigned) g.nodes + 31) >> 5;
+    const unsigned stride = gridDim.x * kBlock;
+    for (unsigned w0 = blockIdx.x * kBlock; w0 < words; w0 += stride) {
+        const unsigned w = w0 + threadIdx.x;
+        const unsigned bits = w < words ? bm[w] : 0u;
+        int total = 0;
+        int excl = block_exclusive_sum(__popc(bits), lds.ep_scan, &total);
+        if (threadIdx.x == 0) lds.ep_base = total ? add_agent(&s.ctl->compact, (unsigned) total) : 0u;
+        __syncthreads();
+        unsigned pos = lds.ep_base + (unsigned) excl;
+        for (unsigned b = bits; b; b &= b - 1) {
+            if (pos < s.cap) out[pos] = (int) ((w << 5) + (unsigned) __builtin_ctz(b));
+            pos++;
+        }
+        __syncthreads();
+    }
 }
 
 template <bool PRED, bool EXACT>
@@ -370,8 +439,7 @@ __device__ void body_contract_expand(const GraphArgs &g, const SearchArgs &s, Cu
     } else {
         hub_phase<PRED>(g, s, c, n, lds, edge);
     }
-    wq.flush();
-    note_depth(s.ctl, labeled, c.iteration);
+    finish_step<PRED>(wq, lds, s.ctl, c.step, labeled, c.iteration, 0u, false);
 }
 
 template <bool PRED, bool EXACT>
@@ -391,7 +459,7 @@ __device__ void body_expand(const GraphArgs &g, const SearchArgs &s, Cursor c, u
     } else {
         hub_phase<PRED>(g, s, c, n, lds, edge);
     }
-    wq.flush();
+    finish_step<PRED>(wq, lds, s.ctl, c.step, false, 0, 0u, false);
 }
 
 template <bool PRED, bool EXACT>
@@ -420,8 +488,7 @@ __device__ void body_contract(const GraphArgs &g, const SearchArgs &s, Cursor c,
         }
         wq.push(keep, u, -1);
     }
-    wq.flush();
-    note_depth(s.ctl, labeled, c.iteration);
+    finish_step<PRED>(wq, lds, s.ctl, c.step, labeled, c.iteration, 0u, false);
 }
 
 template <bool PRED, bool EXACT>
@@ -451,7 +518,7 @@ __device__ void body_filter(const GraphArgs &g, const SearchArgs &s, Cursor c, u
         }
         wq.push(keep, u, p);
     }
-    wq.flush();
+    finish_step<PRED>(wq, lds, s.ctl, c.step, false, 0, 0u, false);
 }
 
 __device__ __forceinline__ bool gathers(int phase)
@@ -536,6 +603,7 @@ __device__ __forceinline__ void prepare_step(Control *ctl, Cursor c, unsigned n)
     st_agent(&ctl->qlen[(c.step + 2) % 3], 0u);
     st_agent(&ctl->steal[(c.step + 1) & 1], 0u);
     st_agent(&ctl->hub_steal[(c.step + 1) & 1], 0u);
+    st_agent(&ctl->compact, 0u);
     __hip_atomic_store(&ctl->hub_alloc[(c.step + 1) & 1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&ctl->fedges[(c.step + 2) % 3], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     ctl->total_queued += n;
@@ -584,6 +652,9 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
                 frontier_bitmap_pass(g, s, c, n, 0);
                 grid_barrier(s.ctl);
                 frontier_bitmap_pass(g, s, c, n, 1);
+                grid_barrier(s.ctl);
+            } else if (dir != kPhaseBottomUp && c.phase == kPhaseBottomUp) {
+                queue_from_bitmap<PRED>(g, s, c, lds);
                 grid_barrier(s.ctl);
             }
             c.phase = dir;
@@ -684,6 +755,12 @@ __global__ __launch_bounds__(kBlock) void hub_kernel(GraphArgs g, SearchArgs s, 
 __global__ __launch_bounds__(kBlock) void frontier_bitmap_kernel(GraphArgs g, SearchArgs s, Cursor c, int pass)
 {
     frontier_bitmap_pass(g, s, c, frontier_length(s, c.step), pass);
+}
+
+__global__ __launch_bounds__(kBlock) void queue_from_bitmap_kernel(GraphArgs g, SearchArgs s, Cursor c)
+{
+    __shared__ Lds<false> lds;
+    queue_from_bitmap<false>(g, s, c, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -823,6 +900,14 @@ hipError_t launch_reset_labels(int *depth, int *pred, unsigned *visited, int nod
 hipError_t launch_init_search(const SearchArgs &s, const int *row, int src, int label_source, hipStream_t st)
 {
     hipLaunchKernelGGL(init_search_kernel, dim3(1), dim3(256), 0, st, s, row, src, label_source);
+    return hipGetLastError();
+}
+
+hipError_t launch_queue_from_bitmap(const SearchArgs &s, Cursor c, int grid, hipStream_t st, int *launches)
+{
+    GraphArgs g{nullptr, nullptr, s.nodes, nullptr, nullptr};
+    hipLaunchKernelGGL(queue_from_bitmap_kernel, dim3(grid), dim3(kBlock), 0, st, g, s, c);
+    (*launches)++;
     return hipGetLastError();
 }
 
