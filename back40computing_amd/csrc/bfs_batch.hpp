@@ -32,6 +32,8 @@ struct alignas(128) BatchSlot {
     unsigned long long fedges[3];
     unsigned long long visited_edges;
     unsigned long long t_begin, t_end;  // wall clock at seeding / completion
+    unsigned ovf[3];          // per-step overflow of the output queue, by step % 3
+    unsigned long long vbase; // sum of fedges over the steps before the current one
 };
 
 struct alignas(256) BatchControl {

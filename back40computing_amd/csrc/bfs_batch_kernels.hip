@@ -27,6 +27,10 @@ struct BatchLds {
     int alive[kMaxSlots];
     int prefix[kMaxSlots + 1];
     int wave_tot[kWaves];
+    // visited-edge sums per query: through the previous step (vsum) and
+    // through the step being planned (vtop)
+    unsigned long long vsum[kMaxSlots];
+    unsigned long long vtop[kMaxSlots];
 };
 
 __device__ __forceinline__ unsigned next_item_batch(unsigned *counter, Lds<false> &lds, unsigned k = 1u)
@@ -42,13 +46,16 @@ __device__ __forceinline__ unsigned long long ld64(const unsigned long long *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Direction rule of the direction-optimizing search, per query.
+// Direction rule of the direction-optimizing search, per query.  ve is the
+// visited-edge sum through this step, built from counters that stay fixed
+// during the level (the running visited_edges total changes while other
+// workgroups already work on the level, and every workgroup must plan the
+// same directions).
 __device__ __forceinline__ int batch_direction(const BatchArgs &a, const BatchSlot &sl, const BatchGraph &g,
-                                               int phase, int step, unsigned n)
+                                               int phase, int step, unsigned n, unsigned long long ve)
 {
     if (!a.direction) return kPhaseExpandContract;
     unsigned long long mf = ld64(&sl.fedges[step % 3]);
-    unsigned long long ve = ld64(&sl.visited_edges);
     unsigned long long mu = g.edges > ve ? g.edges - ve : 0ull;
     if (phase == kPhaseExpandContract) {
         if (mf * (unsigned long long) a.do_alpha > mu) return kPhaseBottomUp;
@@ -75,13 +82,16 @@ __device__ int batch_plan(const BatchArgs &a, BatchLds &L, int level)
             int step = L.step[tid];
             unsigned len = ld_agent(&sl.qlen[step % 3]);
             n = (int) (len < sl.cap ? len : sl.cap);
-            if (ld_agent((const unsigned *) &sl.overflow)) n = 0;
+            // the previous level's overflow flag: fixed during this level
+            if (ld_agent(&sl.ovf[step % 3])) n = 0;
+            L.vtop[tid] = L.vsum[tid] + ld64(&sl.fedges[step % 3]);
             if (n > 0) {
-                dir = batch_direction(a, sl, g, L.phase[tid], step, (unsigned) n);
+                dir = batch_direction(a, sl, g, L.phase[tid], step, (unsigned) n, L.vtop[tid]);
                 const int bu = kBlock * a.bu_tile;
                 tiles = dir == kPhaseBottomUp ? (g.nodes + bu - 1) / bu : (n + kBlock - 1) / kBlock;
                 if (blockIdx.x == 0) {
                     st_agent(&sl.qlen[(step + 2) % 3], 0u);
+                    st_agent(&sl.ovf[(step + 2) % 3], 0u);
                     __hip_atomic_store(&sl.fedges[(step + 2) % 3], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
             } else {
@@ -164,6 +174,7 @@ __device__ __forceinline__ void out_open(SlotOut &o, const BatchArgs &a, BatchLd
     o.wq.out_len = &sl.qlen[(step + 1) % 3];
     o.wq.cap = sl.cap;
     o.wq.overflow = (unsigned *) &sl.overflow;
+    o.wq.overflow_step = &sl.ovf[(step + 1) % 3];
     o.fe = 0;
     o.q = q;
     o.step = step;
@@ -200,7 +211,7 @@ __device__ void batch_td_tile(const BatchArgs &a, BatchLds &L, int q, unsigned l
     unsigned i = local * kBlock + threadIdx.x;
     int v = -1, beg = 0, end = 0;
     if (i < n) {
-        v = sl.q[step & 1][i];
+        v = ld_queue(sl.q[step & 1] + i);
         beg = g.row[v];
         end = g.row[v + 1];
     }
@@ -401,6 +412,7 @@ __device__ void batch_load_cursors(const BatchArgs &a, BatchLds &L)
         L.it[tid] = ld_agent(&sl.iteration);
         L.phase[tid] = ld_agent(&sl.phase);
         L.alive[tid] = ld_agent(&sl.active) && !ld_agent(&sl.done);
+        L.vsum[tid] = ld64(&sl.vbase);
     }
     __syncthreads();
 }
@@ -426,6 +438,7 @@ __global__ __launch_bounds__(kBlock) void batch_persistent_kernel(BatchArgs a)
             L.step[tid]++;
             L.it[tid]++;
             L.phase[tid] = L.dir[tid];
+            L.vsum[tid] = L.vtop[tid];
         }
         __syncthreads();
         level++;
@@ -437,6 +450,7 @@ __global__ __launch_bounds__(kBlock) void batch_persistent_kernel(BatchArgs a)
             sl.step = L.step[tid];
             sl.iteration = L.it[tid];
             sl.phase = L.phase[tid];
+            sl.vbase = L.vsum[tid];
         }
         if (tid == 0) a.ctl->level = level;
     }
@@ -473,6 +487,7 @@ __global__ __launch_bounds__(kBlock) void batch_advance_kernel(BatchArgs a, int 
         BatchSlot &sl = a.slots[tid];
         if (sl.ran) {
             sl.phase = sl.next_phase;
+            sl.vbase += sl.fedges[sl.step % 3];
             sl.step++;
             sl.iteration++;
             sl.ran = 0;
@@ -524,6 +539,8 @@ __global__ void batch_seed_kernel(BatchArgs a, const int *list, int count)
     sl.iteration = 0;
     sl.phase = kPhaseExpandContract;
     sl.overflow = 0;
+    sl.ovf[0] = sl.ovf[1] = sl.ovf[2] = 0;
+    sl.vbase = 0;
     sl.ran = 0;
     sl.done = 0;
     sl.t_end = 0;

@@ -51,7 +51,9 @@ struct alignas(256) Control {
     // Direction-optimizing bookkeeping (second 128-byte line).
     unsigned long long fedges[3];     // sum of degrees of each frontier, by step % 3
     unsigned long long visited_edges; // sum of degrees of all vertices labelled so far
-    unsigned long long pad2[12];
+    unsigned long long fast_vsum;     // fast-path hand-off of the per-workgroup visited-edge sum
+    unsigned ovf[4];                  // per-step overflow of the output queue, by step % 3
+    unsigned long long pad2[9];
     unsigned bar_root;        // own 128-byte line
     unsigned pad3[31];
     unsigned bar_leaf[kBarrierFanIn * 32];     // one 128-byte line per leaf counter
@@ -135,6 +137,14 @@ __device__ __forceinline__ unsigned ld_agent(const unsigned *p)
 __device__ __forceinline__ int ld_agent(const int *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Queue entries another workgroup stored in an earlier step, read once:
+// system scope makes the load miss in every cache level, so a line this XCD
+// still holds from an older use of the buffer (an earlier step or search)
+// cannot be returned.  Streamed data, so nothing is lost by not caching it.
+__device__ __forceinline__ int ld_queue(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 __device__ __forceinline__ void st_agent(unsigned *p, unsigned v)
 {
@@ -245,6 +255,7 @@ struct WaveQueue {
     unsigned *out_len;
     unsigned cap;
     unsigned *overflow;
+    unsigned *overflow_step;   // per-step flag the level loop decides on (may alias overflow)
 
     __device__ __forceinline__ void flush()
     {
@@ -257,7 +268,10 @@ struct WaveQueue {
         base = (unsigned) uniform((int) base);
         unsigned lim = 0;
         if (base < cap) lim = min((unsigned) count, cap - base);
-        if (lim < (unsigned) count && lane == 0) st_agent(overflow, 1u);
+        if (lim < (unsigned) count && lane == 0) {
+            st_agent(overflow, 1u);
+            st_agent(overflow_step, 1u);
+        }
         for (unsigned i = lane; i < lim; i += kWave) {
             out[base + i] = buf[i];
             if (PRED) pout[base + i] = pbuf[i];
@@ -432,6 +446,12 @@ __device__ __forceinline__ void gather_rows(int beg, int end, int par, const int
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void grid_barrier(Control *ctl)
 {
+    // Every wave drains its own outstanding memory operations first:
+    // __syncthreads() alone does not wait for other waves' stores, and the
+    // single write-back below (thread 0's release fence) must find all of
+    // this workgroup's stores already in L2, or a store still in flight
+    // stays dirty in this XCD's L2 and readers elsewhere see old memory.
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
         const unsigned nblocks = gridDim.x;
@@ -472,6 +492,8 @@ __device__ __forceinline__ void grid_barrier(Control *ctl)
                 __builtin_amdgcn_s_sleep(2);
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        // the cache invalidation completes before any wave's next load
+        __builtin_amdgcn_s_waitcnt(0);
     }
     __syncthreads();
 }

@@ -100,6 +100,7 @@ __device__ __forceinline__ WaveQueue<PRED> make_queue(const SearchArgs &s, int s
     wq.out_len = &s.ctl->qlen[(step + 1) % 3];
     wq.cap = s.cap;
     wq.overflow = &s.ctl->overflow;
+    wq.overflow_step = &s.ctl->ovf[(step + 1) % 3];
     return wq;
 }
 
@@ -172,7 +173,10 @@ __device__ void finish_step(WaveQueue<PRED> &wq, Lds<PRED> &lds, Control *ctl, i
     for (int w = 0; w < wave; w++) off += lds.ep_count[w];
     const unsigned cnt = (unsigned) wq.count;
     const unsigned lim = off < wq.cap ? min(cnt, wq.cap - off) : 0u;
-    if (lim < cnt && lane == 0) st_agent(wq.overflow, 1u);
+    if (lim < cnt && lane == 0) {
+        st_agent(wq.overflow, 1u);
+        st_agent(wq.overflow_step, 1u);
+    }
     for (unsigned i = (unsigned) lane; i < lim; i += kWave) {
         wq.out[off + i] = wq.buf[i];
         if (PRED) wq.pout[off + i] = wq.pbuf[i];
@@ -276,7 +280,7 @@ __device__ void body_expand_contract(const GraphArgs &g, const SearchArgs &s, Cu
     };
     if (sub == 0) {
         tile_phase<PRED>(g, s, c, n, hub_list(s, c.step, hubs, n), lds, [&](unsigned i, int &beg, int &end, int &par) {
-            int v = in[i];
+            int v = ld_queue(in + i);
             beg = g.row[v];
             end = g.row[v + 1];
             par = v;
@@ -320,20 +324,33 @@ __device__ void body_bottom_up(const GraphArgs &g, const SearchArgs &s, Cursor c
     const unsigned gw = blockIdx.x * kWaves + threadIdx.x / kWave;
     const unsigned per = (groups + nwaves - 1) / nwaves;
     const unsigned g_end = min(groups, (gw + 1) * per);
-    for (unsigned gi = gw * per; gi < g_end; gi++) {
+    // Software pipeline: the visited word and row bounds of the next group
+    // are in flight while the current group scans, which takes one round
+    // trip off every group's dependent chain (row bounds are loaded for
+    // visited vertices too: they are contiguous and cheap).
+    unsigned gi = gw * per;
+    unsigned vis_n = ~0u;
+    int beg_n = 0, end_n = 0;
+    auto load_group = [&](unsigned gg) {
+        const int vv = (int) (gg * kWave + (unsigned) lane);
+        const bool ok = vv < g.nodes;
+        vis_n = ok ? s.visited[(unsigned) vv >> 5] : ~0u;
+        beg_n = ok ? g.rrow[vv] : 0;
+        end_n = ok ? g.rrow[vv + 1] : 0;
+    };
+    if (gi < g_end) load_group(gi);
+    for (; gi < g_end; gi++) {
+        const unsigned vis = vis_n;
+        int beg = beg_n, end = end_n;
+        if (gi + 1 < g_end) load_group(gi + 1);
         int v = (int) (gi * kWave + (unsigned) lane);
         unsigned w = (unsigned) v >> 5;
         bool valid = v < g.nodes;
-        unsigned vis = valid ? s.visited[w] : ~0u;
         bool todo = valid && ((vis >> (v & 31)) & 1u) == 0;
         // Best-effort top-down levels may have lost visited bits to racing
         // read-modify-writes; the label stays the authority there.
         if (!EXACT && todo) todo = s.depth[v] == -1;
-        int beg = 0, end = 0;
-        if (todo) {
-            beg = g.rrow[v];
-            end = g.rrow[v + 1];
-        }
+        if (!todo) end = beg;
         int len = end - beg;
         int parent = -1;
         if (todo && len < kBuLong) {
@@ -353,17 +370,17 @@ __device__ void body_bottom_up(const GraphArgs &g, const SearchArgs &s, Cursor c
             int l = uniform(__builtin_ctzll(lm));
             lm &= lm - 1;
             int b = readlane(beg, l), e = readlane(end, l);
-            int found = -1;
+            int hit = -1;
             for (int k = b; k < e; k += kWave) {
                 int idx = k + lane;
                 int u = idx < e ? g.rcol[idx] : -1;
                 uint64_t hm = ballot(u >= 0 && in_frontier(u));
                 if (hm) {
-                    found = readlane(u, uniform(__builtin_ctzll(hm)));
+                    hit = readlane(u, uniform(__builtin_ctzll(hm)));
                     break;
                 }
             }
-            if (lane == l) parent = found;
+            if (lane == l) parent = hit;
         }
         bool got = parent >= 0;
         if (got) {
@@ -425,8 +442,8 @@ __device__ void body_contract_expand(const GraphArgs &g, const SearchArgs &s, Cu
     auto edge = [&](bool ok, int w, int par) { wq.push(ok, w, par); };
     if (sub == 0) {
         tile_phase<PRED>(g, s, c, n, hub_list(s, c.step, hubs, n), lds, [&](unsigned i, int &beg, int &end, int &par) {
-            int u = in[i];
-            int p = PRED ? pin[i] : -1;
+            int u = ld_queue(in + i);
+            int p = PRED ? ld_queue(pin + i) : -1;
             if (history_pass<PRED>(lds, u) && claim<EXACT>(s.visited, s.depth, u, s.coherent_filter)) {
                 s.depth[u] = c.iteration;
                 if (PRED) s.pred[u] = p;
@@ -451,7 +468,7 @@ __device__ void body_expand(const GraphArgs &g, const SearchArgs &s, Cursor c, u
     auto edge = [&](bool ok, int w, int par) { wq.push(ok, w, par); };
     if (sub == 0) {
         tile_phase<PRED>(g, s, c, n, hub_list(s, c.step, hubs, n), lds, [&](unsigned i, int &beg, int &end, int &par) {
-            int v = in[i];
+            int v = ld_queue(in + i);
             beg = g.row[v];
             end = g.row[v + 1];
             par = v;
@@ -477,8 +494,8 @@ __device__ void body_contract(const GraphArgs &g, const SearchArgs &s, Cursor c,
         unsigned i = base + threadIdx.x;
         int u = -1, p = -1;
         if (i < n) {
-            u = in[i];
-            if (PRED) p = pin[i];
+            u = ld_queue(in + i);
+            if (PRED) p = ld_queue(pin + i);
         }
         bool keep = u >= 0 && history_pass<PRED>(lds, u) && claim<EXACT>(s.visited, s.depth, u, s.coherent_filter);
         if (keep) {
@@ -505,8 +522,8 @@ __device__ void body_filter(const GraphArgs &g, const SearchArgs &s, Cursor c, u
         unsigned i = base + threadIdx.x;
         int u = -1, p = -1;
         if (i < n) {
-            u = in[i];
-            if (PRED) p = pin[i];
+            u = ld_queue(in + i);
+            if (PRED) p = ld_queue(pin + i);
         }
         // Bitmask only: the history hash is reserved for claims, because a
         // hit there must mean the vertex has already been claimed or found
@@ -554,7 +571,7 @@ __device__ __forceinline__ void frontier_bitmap_pass(const GraphArgs &g, const S
     } else {
         const int *in = s.q[c.step & 1];
         for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-            int u = in[i];
+            int u = ld_queue(in + i);
             __hip_atomic_fetch_or(&bm[u >> 5], 1u << (u & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
@@ -562,10 +579,15 @@ __device__ __forceinline__ void frontier_bitmap_pass(const GraphArgs &g, const S
 
 // Direction choice of a direction-optimizing search at the top of a level
 // (grid-uniform: every input is read after the level barrier).
-__device__ __forceinline__ int choose_direction(const SearchArgs &s, Cursor c, unsigned n)
+//
+// ve is the caller's running sum of fedges over the steps up to this one.
+// The global visited_edges total must not be read here: workgroups already
+// in this level's body keep adding to it, and a workgroup reading it late
+// could choose the other direction -- and then a different sequence of grid
+// barriers than the rest of the grid.
+__device__ __forceinline__ int choose_direction(const SearchArgs &s, Cursor c, unsigned n, unsigned long long ve)
 {
     unsigned long long mf = ld64_agent(&s.ctl->fedges[c.step % 3]);
-    unsigned long long ve = ld64_agent(&s.ctl->visited_edges);
     unsigned long long mu = s.total_edges > ve ? s.total_edges - ve : 0ull;
     if (c.phase == kPhaseExpandContract) {
         if (mf * (unsigned long long) s.do_alpha > mu) return kPhaseBottomUp;
@@ -604,6 +626,7 @@ __device__ __forceinline__ void prepare_step(Control *ctl, Cursor c, unsigned n)
     st_agent(&ctl->steal[(c.step + 1) & 1], 0u);
     st_agent(&ctl->hub_steal[(c.step + 1) & 1], 0u);
     st_agent(&ctl->compact, 0u);
+    st_agent(&ctl->ovf[(c.step + 2) % 3], 0u);
     __hip_atomic_store(&ctl->hub_alloc[(c.step + 1) & 1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&ctl->fedges[(c.step + 2) % 3], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     ctl->total_queued += n;
@@ -637,17 +660,26 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
     init_lds(lds);
     unsigned long long t_begin = s.instrument ? wall_clock64() : 0, busy = 0;
     int reason = kExitDone;
+    // Visited-edge sum up to the current step, kept by every workgroup from
+    // counters that no workgroup changes during the step (fedges[step % 3])
+    // -- see choose_direction.  fedges[0] holds the source's degree.
+    unsigned long long vsum = 0;
 
+    // Every decision at the top of a step reads only values final before
+    // the step began and unchanged during it (the input length, the
+    // previous step's overflow flag, this step's frontier edges), so all
+    // workgroups take the same branch and meet the same grid barriers.
     while (true) {
         unsigned n = frontier_length(s, c.step);
-        if (ld_agent(&s.ctl->overflow)) { reason = kExitOverflow; break; }
+        if (ld_agent(&s.ctl->ovf[c.step % 3])) { reason = kExitOverflow; break; }
         if (n == 0) { reason = kExitDone; break; }
         if (loop == kLoopCE && s.sat_t >= 0 && n > (unsigned) s.sat_t) {
             reason = kExitSaturated;
             break;
         }
         if (loop == kLoopDO) {
-            int dir = choose_direction(s, c, n);
+            vsum += ld64_agent(&s.ctl->fedges[c.step % 3]);
+            int dir = choose_direction(s, c, n, vsum);
             if (dir == kPhaseBottomUp && c.phase != kPhaseBottomUp) {
                 frontier_bitmap_pass(g, s, c, n, 0);
                 grid_barrier(s.ctl);
@@ -673,22 +705,33 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
             grid_barrier(s.ctl);
             if (blockIdx.x == 0) {
                 unsigned long long t0 = s.instrument ? wall_clock64() : 0;
+                unsigned long long fv = vsum;  // sum through the step being run
                 while (true) {
                     if (threadIdx.x == 0) prepare_step(s.ctl, c, n);
                     run_body<PRED, EXACT>(c.phase, g, s, c, n, lds, 0, false);
+                    // the next level reads queue entries other waves just
+                    // stored: each wave's stores must have completed, which
+                    // __syncthreads() alone does not wait for
+                    __builtin_amdgcn_s_waitcnt(0);
                     __syncthreads();
                     c = advance(loop, c);
                     if (threadIdx.x == 0) lds.nb = frontier_length(s, c.step);
                     __syncthreads();
                     n = lds.nb;
-                    if (n == 0 || !fast_level(s, c, n) || ld_agent(&s.ctl->overflow)) break;
+                    if (n == 0 || !fast_level(s, c, n) || ld_agent(&s.ctl->ovf[c.step % 3])) break;
                     if (loop == kLoopCE && s.sat_t >= 0 && n > (unsigned) s.sat_t) break;
-                    if (loop == kLoopDO && choose_direction(s, c, n) != c.phase) break;
+                    if (loop == kLoopDO) {
+                        unsigned long long fv2 = fv + ld64_agent(&s.ctl->fedges[c.step % 3]);
+                        if (choose_direction(s, c, n, fv2) != c.phase) break;
+                        fv = fv2;
+                    }
                 }
                 if (threadIdx.x == 0) {
                     st_agent(&s.ctl->fast_step, c.step);
                     st_agent(&s.ctl->fast_iteration, c.iteration);
                     st_agent(&s.ctl->fast_phase, c.phase);
+                    // excludes the step the loop top runs next (it adds it)
+                    __hip_atomic_store(&s.ctl->fast_vsum, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
                 if (s.instrument) busy += wall_clock64() - t0;
             }
@@ -696,6 +739,7 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
             c.step = ld_agent(&s.ctl->fast_step);
             c.iteration = ld_agent(&s.ctl->fast_iteration);
             c.phase = ld_agent(&s.ctl->fast_phase);
+            vsum = ld64_agent(&s.ctl->fast_vsum);
             continue;
         }
 
