@@ -56,6 +56,8 @@ struct alignas(256) Control {
     unsigned long long pad2[9];
     unsigned bar_root;        // own 128-byte line
     unsigned pad3[31];
+    unsigned bar_gen;         // completed barriers (own 128-byte line)
+    unsigned pad4[31];
     unsigned bar_leaf[kBarrierFanIn * 32];     // one 128-byte line per leaf counter
     unsigned bar_release[kBarrierFanIn * 32];  // one 128-byte line per leaf release word
     unsigned level_len[kLevelLog];             // input length of each step
@@ -460,7 +462,12 @@ __device__ __forceinline__ void grid_barrier(Control *ctl)
         // Workgroups on this leaf: ceil((nblocks - leaf) / fan_in).
         const unsigned leaf_size = (nblocks - leaf + kBarrierFanIn - 1) / kBarrierFanIn;
         unsigned *release = &ctl->bar_release[leaf * 32];
-        unsigned gen = __hip_atomic_load(release, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // The generation comes from the one global word, which only the last
+        // arriver of a barrier advances (after every workgroup has read it
+        // here); waiters then wait for their leaf word to reach gen + 1.
+        // Leaf words of leaves a smaller earlier grid never used may lag:
+        // they are compared with gen + 1, never with each other.
+        const unsigned gen = __hip_atomic_load(&ctl->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         unsigned *lc = &ctl->bar_leaf[leaf * 32];
         unsigned arrived = __hip_atomic_fetch_add(lc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -472,28 +479,18 @@ __device__ __forceinline__ void grid_barrier(Control *ctl)
             if (r == leaves - 1) {
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
                 __hip_atomic_store(&ctl->bar_root, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                // Each leaf's word moves one past its own value: launches
-                // with fewer than kBarrierFanIn workgroups leave the upper
-                // leaves behind, so the words need not agree, and writing
-                // this leaf's gen + 1 everywhere could hand a leaf that is
-                // one generation ahead its current value (its waiters would
-                // never see a change).  Nothing else writes these words
-                // while every workgroup waits here.
-                for (unsigned l = 0; l < leaves; l++) {
-                    unsigned *rw = &ctl->bar_release[l * 32];
-                    unsigned cur = __hip_atomic_load(rw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    __hip_atomic_store(rw, cur + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
+                __hip_atomic_store(&ctl->bar_gen, gen + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (unsigned l = 0; l < leaves; l++)
+                    __hip_atomic_store(&ctl->bar_release[l * 32], gen + 1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
                 last = true;
             }
         }
         if (!last) {
-            while (__hip_atomic_load(release, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
+            while (__hip_atomic_load(release, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != gen + 1)
                 __builtin_amdgcn_s_sleep(2);
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        // the cache invalidation completes before any wave's next load
-        __builtin_amdgcn_s_waitcnt(0);
     }
     __syncthreads();
 }

@@ -7,29 +7,40 @@ Launches with fewer than 16 workgroups advance only the low leaves' words,
 so a later full-grid launch starts with words that disagree.  Publishing
 "the last arriver's leaf value + 1" to every leaf can then hand a leaf that
 is exactly one generation ahead its current value, and its waiters never
-leave.  Publishing "each leaf's own value + 1" always releases everyone.
+leave.  The kernels read the generation from one global word instead and
+wait for their leaf word to equal generation + 1, which releases everyone
+whatever the leaf words held before.
 """
 import itertools
 
 FAN_IN = 16
 
 
-def run_barrier(release, nblocks, last_block, per_leaf_increment):
+def run_barrier(state, nblocks, last_block, protocol):
     """One barrier over nblocks workgroups whose last arrival is last_block.
-    Returns the leaves whose waiters would never be released."""
+    state = {"release": [leaf words], "gen": global word}.  Returns the
+    leaves whose waiters would never be released."""
+    release = state["release"]
     leaves = min(nblocks, FAN_IN)
+    if protocol == "global":
+        gen = state["gen"]                                  # read before arriving
+        state["gen"] = gen + 1
+        for l in range(leaves):
+            release[l] = gen + 1
+        return [l for l in range(leaves) if release[l] != gen + 1]
     seen = {l: release[l] for l in range(leaves)}          # read before arriving
     gen_last = release[last_block % FAN_IN]
     for l in range(leaves):
-        release[l] = release[l] + 1 if per_leaf_increment else gen_last + 1
+        release[l] = release[l] + 1 if protocol == "per_leaf" else gen_last + 1
     return [l for l in range(leaves) if release[l] == seen[l]]
 
 
-def launches(release, grids, last_of, per_leaf_increment):
+def launches(release, grids, last_of, protocol):
+    state = {"release": release, "gen": 0}
     stuck = []
     for k, (nblocks, barriers) in enumerate(grids):
         for i in range(barriers):
-            stuck += run_barrier(release, nblocks, last_of(k, i, nblocks), per_leaf_increment)
+            stuck += run_barrier(state, nblocks, last_of(k, i, nblocks), protocol)
     return stuck
 
 
@@ -38,15 +49,16 @@ def test_old_protocol_can_strand_a_leaf():
     # whose first barrier completes on leaf 1 (one generation behind leaf 0)
     release = [0] * FAN_IN
     grids = [(3, 1), (1, 1), (512, 1)]
-    stuck = launches(release, grids, lambda k, i, n: 1 if n == 512 else 0, per_leaf_increment=False)
+    stuck = launches(release, grids, lambda k, i, n: 1 if n == 512 else 0, protocol="last_leaf")
     assert stuck == [0]
 
 
-def test_per_leaf_increment_always_releases():
+def test_fixed_protocols_always_release():
     # every order of small and large launches, every last-arriving leaf
-    for sizes in itertools.product([1, 3, 7, 16, 512], repeat=3):
-        for last in range(FAN_IN):
-            release = [0] * FAN_IN
-            grids = [(n, b) for n, b in zip(sizes, (1, 2, 3))]
-            stuck = launches(release, grids, lambda k, i, n: last % n, per_leaf_increment=True)
-            assert stuck == []
+    for protocol in ("per_leaf", "global"):
+        for sizes in itertools.product([1, 3, 7, 16, 512], repeat=3):
+            for last in range(FAN_IN):
+                release = [0] * FAN_IN
+                grids = [(n, b) for n, b in zip(sizes, (1, 2, 3))]
+                stuck = launches(release, grids, lambda k, i, n: last % n, protocol=protocol)
+                assert stuck == [], (protocol, sizes, last)
