@@ -232,9 +232,27 @@ __device__ void batch_hub_phase(const BatchArgs &a, BatchLds &L, int level)
     unsigned total = last.first_chunk + (unsigned) ((last.end - last.beg + kHubChunk - 1) / kHubChunk);
     SlotOut o;
     o.q = -1;
+    // strided grabs of chunks: one counter atomic serves up to grab_max chunks
+    const unsigned W = gridDim.x;
+    const unsigned grab = max(1u, min((unsigned) a.grab_max, total / (W * 8u)));
+    unsigned k_left = 0, ch_next = 0;
     while (true) {
-        unsigned ch = next_item_batch(&a.ctl->hub_steal[level & 1], L.base);
-        if (ch >= total) break;
+        if (k_left == 0) {
+            const unsigned i = next_item_batch(&a.ctl->hub_steal[level & 1], L.base);
+            ch_next = (i / W) * W * grab + i % W;
+            k_left = grab;
+            // first chunks of successive grabs only grow: none is left
+            if (ch_next >= total) break;
+        } else {
+            __syncthreads();  // LDS reuse fence between chunks
+        }
+        const unsigned ch = ch_next;
+        ch_next += W;
+        k_left--;
+        if (ch >= total) {  // the rest of this grab is past the end
+            k_left = 0;
+            continue;
+        }
         if (threadIdx.x == 0) {
             unsigned lo = 0, hi = entries - 1;
             while (lo < hi) {

@@ -227,8 +227,11 @@ __device__ __forceinline__ void hub_phase(const GraphArgs &g, const SearchArgs &
     HubEntry lastent = list[entries - 1];
     const int chunk = hub_chunk(n);
     unsigned total = lastent.first_chunk + (unsigned) ((lastent.end - lastent.beg + chunk - 1) / chunk);
+    // guided grabs of consecutive chunks, as for tiles: a level of many hub
+    // chunks must not queue on the one chunk counter
+    TileGrab tg{0u, 0u, max(1u, min((unsigned) s.grab_max, total / (gridDim.x * 8u)))};
     while (true) {
-        unsigned ch = next_item(&s.ctl->hub_steal[c.step & 1], lds);
+        unsigned ch = next_tile(&s.ctl->hub_steal[c.step & 1], lds, tg);
         if (ch >= total) break;
         if (threadIdx.x == 0) {
             // last entry with first_chunk <= ch (entries are in chunk order)
