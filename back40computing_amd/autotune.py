@@ -38,8 +38,8 @@ SPACE_SINGLE = {
     "blocks_per_cu": [1, 2, 4],
     "tile_grab_max": [1, 4, 16, 32, 64],
     "wave_gather_threshold": [32, 64, 128],
-    "cta_gather_threshold": [256, 512, 1024, 2048],
-    "grid_gather_threshold": [4096, 8192, 32768, -1],
+    "cta_gather_threshold": [64, 128, 256, 512, 1024],
+    "grid_gather_threshold": [2048, 4096, 8192, 32768, -1],
     "do_alpha": [2, 4, 8, 14, 24],
     "do_beta": [8, 16, 24, 48],
 }

@@ -270,12 +270,12 @@ bfs_status bfs_batch_config_default(bfs_config *cfg)
     // and a later switch to bottom-up than the single-source default.
     cfg->blocks_per_cu = 4;
     cfg->bu_tile_blocks = 64;
-    cfg->tile_grab_max = 128;
+    cfg->tile_grab_max = 32;
     cfg->grab_strided = 2;
     cfg->grid_gather_threshold = 4096;
     cfg->wave_gather_threshold = 64;
     cfg->cta_gather_threshold = 1024;
-    cfg->do_alpha = 8;
+    cfg->do_alpha = 2;
     cfg->do_beta = 48;
     return BFS_OK;
 }

@@ -49,14 +49,14 @@ bfs_status hip_fail(hipError_t e, const char *what)
 // back40computing_amd/tuned.json).
 const bfs_config kDefaults = {
     /*wave_gather_threshold*/ 64,
-    /*cta_gather_threshold*/ 256,
+    /*cta_gather_threshold*/ 64,
     /*fast_path_threshold*/ 1024,
     /*saturation_threshold*/ 0,   // 0: persistent grid size * 256
     /*blocks_per_cu*/ 2,
     /*cull_mode*/ BFS_CULL_EXACT,
     /*instrument*/ 0,
     /*grid_gather_threshold*/ 8192,
-    /*do_alpha*/ 24,
+    /*do_alpha*/ 8,
     /*do_beta*/ 48,
     /*tile_grab_max*/ 32,
     /*bu_tile_blocks*/ 1,
