@@ -588,16 +588,23 @@ __device__ __forceinline__ void frontier_bitmap_pass(const GraphArgs &g, const S
 // in this level's body keep adding to it, and a workgroup reading it late
 // could choose the other direction -- and then a different sequence of grid
 // barriers than the rest of the grid.
-__device__ __forceinline__ int choose_direction(const SearchArgs &s, Cursor c, unsigned n, unsigned long long ve)
+// Direction rule: mf = the frontier's edges, n = its vertices, ve = edges
+// visited so far.
+__device__ __forceinline__ int direction_rule(const SearchArgs &s, int phase, unsigned long long mf, unsigned n,
+                                              unsigned long long ve)
 {
-    unsigned long long mf = ld64_agent(&s.ctl->fedges[c.step % 3]);
     unsigned long long mu = s.total_edges > ve ? s.total_edges - ve : 0ull;
-    if (c.phase == kPhaseExpandContract) {
+    if (phase == kPhaseExpandContract) {
         if (mf * (unsigned long long) s.do_alpha > mu) return kPhaseBottomUp;
     } else if ((unsigned long long) n * (unsigned long long) s.do_beta < (unsigned long long) s.nodes) {
         return kPhaseExpandContract;
     }
-    return c.phase;
+    return phase;
+}
+
+__device__ __forceinline__ int choose_direction(const SearchArgs &s, Cursor c, unsigned n, unsigned long long ve)
+{
+    return direction_rule(s, c.phase, ld64_agent(&s.ctl->fedges[c.step % 3]), n, ve);
 }
 
 // Whether a level is small enough for the single-workgroup fast path.
@@ -651,6 +658,291 @@ __device__ __forceinline__ bool hubs_pending(const SearchArgs &s, int step)
 }
 
 // ---------------------------------------------------------------------------
+// LDS-resident fast path of the vertex-frontier loops (expand-contract and
+// the top-down levels of direction-optimizing search).  Block 0 alone runs
+// consecutive small levels while the rest of the grid waits at a grid
+// barrier.  The frontier, its members' row starts and degrees stay in LDS
+// between levels, and the global queue and counters are written once, when
+// the path ends.  The level's edges are spread evenly over the threads, each
+// taking kFastK consecutive edges through the exclusive degree scan, and each
+// memory trip of a batch is issued for all kFastK edges before the first
+// result is used: column index, visited word, claim, row bounds -- four
+// dependent trips per batch, and a level of up to kBlock * kFastK edges is
+// one batch.  A high-diameter graph spends its whole search here.
+// ---------------------------------------------------------------------------
+constexpr int kFastCap = 1024;  // frontier members kept in LDS
+constexpr int kFastK = 4;       // consecutive edges per thread per batch
+static_assert(kFastCap <= 4 * kBlock, "the degree scan takes four members per thread");
+
+struct FastFront {
+    int v[2][kFastCap];
+    int beg[2][kFastCap];
+    int deg[2][kFastCap];        // degree on append; the level's exclusive scan in place
+    unsigned count;              // next-frontier appends
+    unsigned long long fe;       // next frontier's edges
+    int labeled;
+    int scan[kWaves];            // per-wave sums of the degree scan
+    unsigned long long wfe[kWaves];
+    int wlab[kWaves];
+};
+
+// The level loop's LDS: the step bodies' staging and the fast path's
+// frontier are never live at once (the fast path starts after a grid
+// barrier and re-initialises the step state when it ends).
+template <bool PRED>
+union LoopLds {
+    Lds<PRED> step;
+    FastFront fast;
+};
+
+// Exclusive scan of F.deg[b][0, n) in place; returns the sum to every thread.
+__device__ int fast_scan(FastFront &F, int b, unsigned n)
+{
+    const unsigned t0 = 4u * threadIdx.x;
+    int d[4], sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        d[k] = t0 + k < n ? F.deg[b][t0 + k] : 0;
+        sum += d[k];
+    }
+    int total;
+    int off = block_exclusive_sum(sum, F.scan, &total);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (t0 + k < n) F.deg[b][t0 + k] = off;
+        off += d[k];
+    }
+    __syncthreads();
+    return total;
+}
+
+// One level from LDS buffer src (n members, 0 < n <= kFastCap) into buffer
+// src ^ 1; members past kFastCap go to the global queue of step c + 1 at the
+// same positions.  Every thread calls it.
+template <bool PRED, bool EXACT>
+__device__ void fast_lds_level(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, int src, FastFront &F)
+{
+    const int dst = src ^ 1;
+    const int lane = lane_id(), wave = threadIdx.x / kWave;
+    const int next_depth = c.iteration + 1;
+    int *spill = s.q[(c.step + 1) & 1];
+    if (threadIdx.x == 0) F.count = 0;  // ordered before the appends by the scan's barriers
+    const int total = fast_scan(F, src, n);
+    const int *offs = F.deg[src];
+    const int *begs = F.beg[src];
+    const int *vs = F.v[src];
+    unsigned long long fe = 0;
+    bool labeled = false;
+    for (int b0 = 0; b0 < total; b0 += kBlock * kFastK) {
+        const int e0 = b0 + (int) threadIdx.x * kFastK;
+        int u[kFastK], pv[kFastK];
+        // member holding edge e0: the last i with offs[i] <= e0 (it has edges)
+        int i = 0;
+        if (e0 < total) {
+            int lo = 0, hi = (int) n - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (offs[mid] <= e0) lo = mid;
+                else hi = mid - 1;
+            }
+            i = lo;
+        }
+        // column positions first (LDS only): a loop between two global
+        // loads makes the compiler wait for the first
+        int at[kFastK];
+#pragma unroll
+        for (int k = 0; k < kFastK; k++) {
+            const int e = e0 + k;
+            at[k] = -1;
+            pv[k] = -1;
+            if (e < total) {
+                while (i + 1 < (int) n && offs[i + 1] <= e) i++;
+                at[k] = begs[i] + (e - offs[i]);
+                pv[k] = vs[i];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kFastK; k++) u[k] = at[k] >= 0 ? __builtin_nontemporal_load(g.col + at[k]) : -1;
+        // visited words (a set bit is final), then claims of the clear ones
+        unsigned cur[kFastK];
+#pragma unroll
+        for (int k = 0; k < kFastK; k++) {
+            cur[k] = ~0u;
+            if (u[k] >= 0) {
+                unsigned *w = s.visited + (u[k] >> 5);
+                cur[k] = s.coherent_filter ? __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *w;
+            }
+        }
+        // The claims' results are read only after all are issued (a use
+        // next to its atomic would wait out each round trip in turn).
+        unsigned old[kFastK];
+#pragma unroll
+        for (int k = 0; k < kFastK; k++) {
+            old[k] = ~0u;
+            const unsigned bit = 1u << (u[k] & 31);
+            if (u[k] >= 0 && !(cur[k] & bit))
+                // agent scope: another XCD's L2 may hold this word from
+                // before an earlier step's claims
+                old[k] = __hip_atomic_fetch_or(s.visited + (u[k] >> 5), bit, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // (the empty asm keeps the compiler from folding a use into the
+        // atomic's own block)
+#pragma unroll
+        for (int k = 0; k < kFastK; k++) asm volatile("" : "+v"(old[k]));
+        unsigned fresh = 0;
+#pragma unroll
+        for (int k = 0; k < kFastK; k++)
+            if (!((old[k] >> (u[k] & 31)) & 1u)) fresh |= 1u << k;
+        if (!EXACT) {
+            // best-effort culling elsewhere may have lost bits of labelled vertices
+            int dep[kFastK];
+#pragma unroll
+            for (int k = 0; k < kFastK; k++) dep[k] = ((fresh >> k) & 1) ? s.depth[u[k]] : -1;
+#pragma unroll
+            for (int k = 0; k < kFastK; k++)
+                if (dep[k] != -1) fresh &= ~(1u << k);
+        }
+        int ub[kFastK], ud[kFastK];
+#pragma unroll
+        for (int k = 0; k < kFastK; k++) {
+            ub[k] = 0;
+            ud[k] = 0;
+            if ((fresh >> k) & 1) {
+                s.depth[u[k]] = next_depth;
+                if (PRED) s.pred[u[k]] = pv[k];
+                ub[k] = g.row[u[k]];
+                ud[k] = g.row[u[k] + 1];
+            }
+        }
+        // append the new members: one LDS reservation per wave
+        const int cnt = __popc(fresh);
+        const int incl = wave_inclusive_sum(cnt);
+        const int wtot = readlane(incl, kWave - 1);
+        if (wtot) {
+            unsigned base = 0;
+            if (lane == 0) base = atomicAdd(&F.count, (unsigned) wtot);
+            unsigned p = (unsigned) readlane((int) base, 0) + (unsigned) (incl - cnt);
+#pragma unroll
+            for (int k = 0; k < kFastK; k++) {
+                if ((fresh >> k) & 1) {
+                    const int d = ud[k] - ub[k];
+                    fe += (unsigned) d;
+                    labeled = true;
+                    if (p < (unsigned) kFastCap) {
+                        F.v[dst][p] = u[k];
+                        F.beg[dst][p] = ub[k];
+                        F.deg[dst][p] = d;
+                    } else if (p < s.cap) {
+                        spill[p] = u[k];
+                    }
+                    p++;
+                }
+            }
+        }
+    }
+    // fold the threads' edge sums and label flags
+    const int lo = wave_inclusive_sum((int) (fe & 0xFFFFu));
+    const int hi = wave_inclusive_sum((int) (fe >> 16));
+    const bool any = ballot(labeled) != 0;
+    if (lane == kWave - 1) {
+        F.wfe[wave] = ((unsigned long long) (unsigned) hi << 16) + (unsigned long long) (unsigned) lo;
+        F.wlab[wave] = any ? 1 : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long f = 0;
+        int l = 0;
+        for (int w = 0; w < kWaves; w++) {
+            f += F.wfe[w];
+            l |= F.wlab[w];
+        }
+        F.fe = f;
+        F.labeled = l;
+    }
+    __syncthreads();
+}
+
+// Runs LDS fast levels from step c (input: n <= kFastCap members in the
+// global queue) until a level's output is empty, too large for the path, or
+// (direction-optimizing) calls for bottom-up.  Leaves the global queue,
+// counters and the fast_* hand-off exactly as the ordinary steps would have.
+// `fv`: visited-edge sum through step c (see choose_direction).  Block 0
+// only; every thread calls it.
+template <bool PRED, bool EXACT>
+__device__ void fast_lds_path(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, int loop,
+                              unsigned long long fv, FastFront &F)
+{
+    const int *in = s.q[c.step & 1];
+    for (unsigned i = threadIdx.x; i < n; i += kBlock) {
+        const int v = ld_queue(in + i);
+        const int b = g.row[v];
+        F.v[0][i] = v;
+        F.beg[0][i] = b;
+        F.deg[0][i] = g.row[v + 1] - b;
+    }
+    __syncthreads();
+    int src = 0;
+    unsigned long long ve_add = 0, queued = 0;
+    int maxd = -1;
+    unsigned n2 = 0;
+    unsigned long long fe2 = 0;
+    while (true) {
+        if (threadIdx.x == 0 && c.step < kLevelLog) {
+            s.ctl->level_len[c.step] = n;
+            s.ctl->level_clock[c.step] = wall_clock64();
+        }
+        queued += n;
+        fast_lds_level<PRED, EXACT>(g, s, c, n, src, F);
+        n2 = F.count;
+        fe2 = F.fe;
+        if (F.labeled) maxd = c.iteration + 1;
+        ve_add += fe2;
+        src ^= 1;
+        c = advance(loop, c);
+        bool go = n2 > 0 && n2 <= (unsigned) kFastCap && n2 <= (unsigned) s.fast_t &&
+                  fe2 <= 32ull * (unsigned) s.fast_t;
+        if (go && loop == kLoopDO) {
+            const unsigned long long fv2 = fv + fe2;
+            if (direction_rule(s, c.phase, fe2, n2, fv2) != c.phase) go = false;
+            else fv = fv2;
+        }
+        n = n2;
+        if (!go) break;
+        __syncthreads();  // every thread has read F's results before the next level resets them
+    }
+    // The next step (c) reads its input from the global queue: copy the
+    // LDS-resident members (those past kFastCap were written there already).
+    const unsigned held = min(n2, (unsigned) kFastCap);
+    int *q = s.q[c.step & 1];
+    for (unsigned i = threadIdx.x; i < held && i < s.cap; i += kBlock) q[i] = F.v[src][i];
+    if (threadIdx.x == 0) {
+        Control *ctl = s.ctl;
+        // what the previous step's prepare_step would have left for step c
+        st_agent(&ctl->qlen[(c.step + 1) % 3], 0u);
+        st_agent(&ctl->steal[c.step & 1], 0u);
+        st_agent(&ctl->hub_steal[c.step & 1], 0u);
+        st_agent(&ctl->compact, 0u);
+        st_agent(&ctl->ovf[(c.step + 1) % 3], 0u);
+        __hip_atomic_store(&ctl->hub_alloc[c.step & 1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctl->fedges[(c.step + 1) % 3], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // step c's input
+        st_agent(&ctl->qlen[c.step % 3], min(n2, s.cap));
+        st_agent(&ctl->ovf[c.step % 3], n2 > s.cap ? 1u : 0u);
+        if (n2 > s.cap) st_agent(&ctl->overflow, 1u);
+        __hip_atomic_store(&ctl->fedges[c.step % 3], fe2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&ctl->visited_edges, ve_add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (maxd >= 0) atomicMax(&ctl->max_depth, maxd);
+        ctl->total_queued += queued;
+        st_agent(&ctl->fast_step, c.step);
+        st_agent(&ctl->fast_iteration, c.iteration);
+        st_agent(&ctl->fast_phase, c.phase);
+        // excludes the step the loop top runs next (it adds it)
+        __hip_atomic_store(&ctl->fast_vsum, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Persistent level loop (cooperative launch).
 // ---------------------------------------------------------------------------
 // One instantiation per strategy (LOOP), so profiles attribute time to the
@@ -659,7 +951,8 @@ template <bool PRED, bool EXACT, int LOOP>
 __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchArgs s, Cursor c)
 {
     constexpr int loop = LOOP;
-    __shared__ Lds<PRED> lds;
+    __shared__ LoopLds<PRED> shared;
+    Lds<PRED> &lds = shared.step;
     init_lds(lds);
     unsigned long long t_begin = s.instrument ? wall_clock64() : 0, busy = 0;
     int reason = kExitDone;
@@ -695,7 +988,8 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
             c.phase = dir;
         }
 
-        if (fast_level(s, c, n) && gridDim.x > 1) {
+        if (fast_level(s, c, n) && gridDim.x > 1 &&
+            ((LOOP != kLoopEC && LOOP != kLoopDO) || n <= (unsigned) kFastCap)) {
             // Single-workgroup fast path: block 0 runs consecutive small
             // levels with workgroup barriers only (no grid tier: it alone
             // works); the rest of the grid waits at the next grid barrier.
@@ -706,7 +1000,16 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
             // an empty frontier and leave the loop, stranding the grid at
             // the barrier below.
             grid_barrier(s.ctl);
-            if (blockIdx.x == 0) {
+            if constexpr (LOOP == kLoopEC || LOOP == kLoopDO) {
+                // frontier kept in LDS between levels
+                if (blockIdx.x == 0) {
+                    unsigned long long t0 = s.instrument ? wall_clock64() : 0;
+                    fast_lds_path<PRED, EXACT>(g, s, c, n, loop, vsum, shared.fast);
+                    __syncthreads();
+                    init_lds(lds);
+                    if (s.instrument) busy += wall_clock64() - t0;
+                }
+            } else if (blockIdx.x == 0) {
                 unsigned long long t0 = s.instrument ? wall_clock64() : 0;
                 unsigned long long fv = vsum;  // sum through the step being run
                 while (true) {
