@@ -94,6 +94,7 @@ struct bfs_problem {
 
     Control *d_ctl = nullptr;
     Control *h_ctl = nullptr;    // pinned mirror
+    Control *h_ctl_dev = nullptr;  // its device-side address
     HubEntry *d_hubs = nullptr;  // two grid-tier lists of kHubCapacity entries
     unsigned *d_fbm = nullptr;   // two frontier bitmaps (bottom-up levels)
     int wall_khz = 100000;
@@ -158,13 +159,17 @@ SearchArgs search_args(const bfs_problem *p, int grid)
     s.total_edges = (unsigned long long) p->edges;
     s.track_edges = 1;
     s.do_alpha = p->cfg.do_alpha;
+    s.init_src = -1;
+    s.init_reset = 0;
+    s.init_label = 0;
+    s.host_head = nullptr;
     s.do_beta = p->cfg.do_beta;
     return s;
 }
 
 bfs_status read_control(bfs_problem *p)
 {
-    HIP_TRY(hipMemcpyAsync(p->h_ctl, p->d_ctl, 256, hipMemcpyDeviceToHost, p->stream), "read control block");
+    HIP_TRY(hipMemcpyAsync(p->h_ctl, p->d_ctl, kControlHead, hipMemcpyDeviceToHost, p->stream), "read control block");
     HIP_TRY(hipStreamSynchronize(p->stream), "synchronize");
     return BFS_OK;
 }
@@ -445,6 +450,11 @@ bfs_status bfs_problem_create(const int32_t *h_row_offsets, const int32_t *h_col
         return cleanup(hip_fail(e, "allocate predecessors"));
     if ((e = hipHostMalloc(&p->h_ctl, sizeof(Control), hipHostMallocDefault)) != hipSuccess)
         return cleanup(hip_fail(e, "allocate pinned control block"));
+    if ((e = hipHostGetDevicePointer(reinterpret_cast<void **>(&p->h_ctl_dev), p->h_ctl, 0)) != hipSuccess)
+        return cleanup(hip_fail(e, "map pinned control block"));
+    // the grid barrier's counters must start at zero (searches leave them so)
+    if ((e = hipMemset(p->d_ctl, 0, sizeof(Control))) != hipSuccess)
+        return cleanup(hip_fail(e, "clear control block"));
     if ((e = hipMemcpy(p->d_row, h_row_offsets, sizeof(int) * ((size_t) nodes + 1), hipMemcpyHostToDevice)) != hipSuccess)
         return cleanup(hip_fail(e, "copy row offsets"));
     if (edges > 0 &&
@@ -545,9 +555,7 @@ bfs_status bfs_enact(bfs_problem *p, bfs_strategy strategy, int32_t fused, int32
     if (p->frontier_type != need && p->frontier_type != BFS_MIXED_FRONTIERS)
         return fail(BFS_ERR_INVALID_ARG, "problem was reset for another frontier type");
     HIP_TRY(hipSetDevice(p->device), "hipSetDevice");
-    if (p->dirty) {
-        HIP_TRY(launch_reset_labels(p->d_depth, p->d_pred, p->d_visited, p->nodes, p->stream), "reset labels");
-    }
+    const bool reset = p->dirty;
     p->dirty = true;
 
     const bool exact = p->cfg.cull_mode == BFS_CULL_EXACT;
@@ -566,7 +574,19 @@ bfs_status bfs_enact(bfs_problem *p, bfs_strategy strategy, int32_t fused, int32
 
     int launches = 0;
     if (!vertex_frontier) s.track_edges = 0;
-    HIP_TRY(launch_init_search(s, p->d_row, src, vertex_frontier ? 1 : 0, p->stream), "init search");
+    // One persistent launch per search starts the search itself and leaves
+    // the status head in pinned memory; the other paths start it with a
+    // launch of its own and copy the head afterwards.
+    const bool folded = fused && strategy != BFS_HYBRID;
+    if (folded) {
+        s.init_src = src;
+        s.init_reset = reset ? 1 : 0;
+        s.init_label = vertex_frontier ? 1 : 0;
+        // instrumented cycle sums arrive after the kernel's copy of the head
+        if (!s.instrument) s.host_head = p->h_ctl_dev;
+    } else {
+        HIP_TRY(launch_init_search(s, p->d_row, src, vertex_frontier ? 1 : 0, reset, p->stream), "init search");
+    }
     HIP_TRY(hipEventRecord(p->ev_start, p->stream), "record start");
     bfs_status st = BFS_OK;
     Cursor c{0, 0, kPhaseExpandContract};
@@ -613,10 +633,14 @@ bfs_status bfs_enact(bfs_problem *p, bfs_strategy strategy, int32_t fused, int32
     }
     if (st != BFS_OK) return st;
     HIP_TRY(hipEventRecord(p->ev_stop, p->stream), "record stop");
-    HIP_TRY(hipEventSynchronize(p->ev_stop), "synchronize stop");
+    // the status head of the control block, queued behind the search: one
+    // wait covers both
+    if (!s.host_head)
+        HIP_TRY(hipMemcpyAsync(p->h_ctl, p->d_ctl, kControlHead, hipMemcpyDeviceToHost, p->stream),
+                "read control block");
+    HIP_TRY(hipStreamSynchronize(p->stream), "synchronize");
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, p->ev_start, p->ev_stop), "elapsed time");
-    HIP_TRY(hipMemcpy(p->h_ctl, p->d_ctl, sizeof(Control), hipMemcpyDeviceToHost), "read control block");
     const Control &h = *p->h_ctl;
     if (stats) {
         memset(stats, 0, sizeof(*stats));
@@ -687,6 +711,8 @@ bfs_status bfs_get_level_log(const bfs_problem *p, int32_t max_steps, uint32_t *
                              int32_t *steps)
 {
     if (!p || !steps || max_steps < 0) return fail(BFS_ERR_INVALID_ARG, "invalid argument");
+    HIP_TRY(hipSetDevice(p->device), "hipSetDevice");
+    HIP_TRY(hipMemcpy(p->h_ctl, p->d_ctl, sizeof(Control), hipMemcpyDeviceToHost), "read control block");
     const Control &h = *p->h_ctl;
     int k = 0;
     for (; k < kLevelLog && k < max_steps; k++) {

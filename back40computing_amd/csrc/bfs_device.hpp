@@ -10,6 +10,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 namespace b40c_hip {
@@ -63,6 +64,11 @@ struct alignas(256) Control {
     unsigned level_len[kLevelLog];             // input length of each step
     unsigned long long level_clock[kLevelLog]; // wall clock at the start of each step
 };
+
+// Bytes of the control block's status head (counters, flags, statistics):
+// what the host reads after a search or a step.
+constexpr size_t kControlHead = 256;
+static_assert(offsetof(Control, bar_root) == kControlHead, "status head ends where the barrier lines begin");
 
 // Grid-tier work item: a row too long for one workgroup, split into
 // chunk-edge chunks [first_chunk, first_chunk + ceil(len / chunk)); the

@@ -48,6 +48,16 @@ struct SearchArgs {
     int instrument;
     int grab_max;          // most tiles taken per tile-counter atomic
     int coherent_filter;   // claim pre-test reads the device-coherent visited word
+    // Search start folded into a persistent launch (init_src >= 0): the
+    // kernel clears the status head, queues init_src, labels it
+    // (init_label) and, with init_reset, rewrites every label first --
+    // what launch_init_search does as a launch of its own.
+    int init_src;
+    int init_reset;
+    int init_label;
+    // Pinned host copy of the control block's status head, written by a
+    // persistent launch when its level loop ends (null: none).
+    Control *host_head;
 };
 
 // Loop position: step index (selects queues and counters), BFS iteration
@@ -62,7 +72,10 @@ struct Cursor {
 enum LoopKind : int { kLoopEC = 0, kLoopCE = 1, kLoopTP = 2, kLoopDO = 3 };
 
 // Host launchers (bfs_kernels.hip).
-hipError_t launch_init_search(const SearchArgs &s, const int *row, int src, int label_source, hipStream_t st);
+// Clears the control block and queues src; with reset, also rewrites every
+// label (depth, predecessor, visited bit) in the same launch.
+hipError_t launch_init_search(const SearchArgs &s, const int *row, int src, int label_source, bool reset,
+                              hipStream_t st);
 // Sets the frontier bitmap fbm[iteration & 1] from the queue of step c.
 hipError_t launch_build_frontier_bitmap(const SearchArgs &s, Cursor c, int grid, hipStream_t st, int *launches);
 // Bitmap fbm[c.iteration & 1] -> input queue of step c (bottom-up -> top-down).

@@ -22,6 +22,8 @@
 #include "bfs_device.hpp"
 #include "bfs_internal.hpp"
 
+#include <algorithm>
+
 namespace b40c_hip {
 
 template <bool PRED>
@@ -608,11 +610,17 @@ __device__ __forceinline__ int choose_direction(const SearchArgs &s, Cursor c, u
 }
 
 // Whether a level is small enough for the single-workgroup fast path.
-__device__ __forceinline__ bool fast_level(const SearchArgs &s, Cursor c, unsigned n)
+// fe: the step's frontier edges (used when they are tracked).
+__device__ __forceinline__ bool fast_level(const SearchArgs &s, Cursor c, unsigned n, unsigned long long fe)
 {
     if (s.fast_t < 0 || n > (unsigned) s.fast_t || c.phase == kPhaseBottomUp) return false;
-    if (s.track_edges && ld64_agent(&s.ctl->fedges[c.step % 3]) > 32ull * (unsigned) s.fast_t) return false;
+    if (s.track_edges && fe > 32ull * (unsigned) s.fast_t) return false;
     return true;
+}
+
+__device__ __forceinline__ bool fast_level(const SearchArgs &s, Cursor c, unsigned n)
+{
+    return fast_level(s, c, n, s.track_edges ? ld64_agent(&s.ctl->fedges[c.step % 3]) : 0ull);
 }
 
 __device__ __forceinline__ Cursor advance(int loop, Cursor c)
@@ -943,6 +951,74 @@ __device__ void fast_lds_path(const GraphArgs &g, const SearchArgs &s, Cursor c,
 }
 
 // ---------------------------------------------------------------------------
+// Search start and end inside a persistent launch (one launch per search).
+// ---------------------------------------------------------------------------
+
+// Every workgroup rewrites its share of the labels (with init_reset); block 0
+// clears the status head and the level log and queues the source.  Only the
+// labels and block 0's own words are written, so the first step may begin
+// before a grid barrier as long as it reads nothing of the control block
+// and claims nothing before one (the loop top takes step 0's values from
+// search_start_values).
+__device__ void search_prologue(const GraphArgs &g, const SearchArgs &s)
+{
+    const int src = s.init_src;
+    if (s.init_reset) {
+        const int words = (s.nodes + 31) >> 5;
+        for (int i = blockIdx.x * kBlock + threadIdx.x; i < s.nodes; i += gridDim.x * kBlock) {
+            const bool is_src = s.init_label && i == src;
+            s.depth[i] = is_src ? 0 : -1;
+            if (s.pred) s.pred[i] = is_src ? -2 : -1;
+            if (i < words) s.visited[i] = (s.init_label && (src >> 5) == i) ? 1u << (src & 31) : 0u;
+        }
+    }
+    if (blockIdx.x != 0) return;
+    Control *ctl = s.ctl;
+    unsigned *head = reinterpret_cast<unsigned *>(ctl);
+    for (int i = threadIdx.x; i < (int) (kControlHead / 4); i += kBlock) head[i] = 0u;
+    for (int i = threadIdx.x; i < kLevelLog; i += kBlock) {
+        ctl->level_len[i] = 0u;
+        ctl->level_clock[i] = 0ull;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ctl->qlen[0] = 1;
+        ctl->max_depth = -1;
+        s.q[0][0] = src;
+        if (s.pq[0]) s.pq[0][0] = -2;
+        if (s.init_label) {
+            if (!s.init_reset) {
+                s.depth[src] = 0;
+                if (s.pred) s.pred[src] = -2;
+                s.visited[src >> 5] |= 1u << (src & 31);
+            }
+            ctl->max_depth = 0;
+            const unsigned long long deg = (unsigned long long) (g.row[src + 1] - g.row[src]);
+            ctl->fedges[0] = deg;
+            ctl->visited_edges = deg;
+        }
+    }
+}
+
+// Block 0 copies the status head to s.host_head when the level loop ends,
+// so the host reads it after the launch without a copy of its own.  The
+// loop ends at a loop top, after the grid barrier that completed the last
+// step's counter updates, and the exit fields are block 0's own: the head
+// is final (the instrumented cycle sums, added later, are not copied).
+// Called by thread 0 of block 0.
+__device__ void publish_status(const SearchArgs &s)
+{
+    const unsigned *head = reinterpret_cast<const unsigned *>(s.ctl);
+    unsigned *out = reinterpret_cast<unsigned *>(s.host_head);
+    unsigned w[kControlHead / 4];
+#pragma unroll
+    for (int i = 0; i < (int) (kControlHead / 4); i++) w[i] = __hip_atomic_load(head + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int i = 0; i < (int) (kControlHead / 4); i++) out[i] = w[i];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");  // system scope: the host reads it next
+}
+
+// ---------------------------------------------------------------------------
 // Persistent level loop (cooperative launch).
 // ---------------------------------------------------------------------------
 // One instantiation per strategy (LOOP), so profiles attribute time to the
@@ -961,21 +1037,40 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
     // -- see choose_direction.  fedges[0] holds the source's degree.
     unsigned long long vsum = 0;
 
+    // With a folded search start, step 0's loop-top values come from the
+    // graph (one source, its degree), not from the control block that block
+    // 0 is still writing, and the step's first grid barrier orders the
+    // label rewrite before any claim.
+    bool start = s.init_src >= 0;
+    if (start) search_prologue(g, s);
+
     // Every decision at the top of a step reads only values final before
     // the step began and unchanged during it (the input length, the
     // previous step's overflow flag, this step's frontier edges), so all
     // workgroups take the same branch and meet the same grid barriers.
     while (true) {
-        unsigned n = frontier_length(s, c.step);
-        if (ld_agent(&s.ctl->ovf[c.step % 3])) { reason = kExitOverflow; break; }
-        if (n == 0) { reason = kExitDone; break; }
+        unsigned n;
+        unsigned long long fe = 0;
+        if (start) {
+            n = 1;
+            if (s.track_edges && s.init_label) fe = (unsigned long long) (g.row[s.init_src + 1] - g.row[s.init_src]);
+        } else {
+            n = frontier_length(s, c.step);
+            if (ld_agent(&s.ctl->ovf[c.step % 3])) { reason = kExitOverflow; break; }
+            if (n == 0) { reason = kExitDone; break; }
+            if (s.track_edges) fe = ld64_agent(&s.ctl->fedges[c.step % 3]);
+        }
         if (loop == kLoopCE && s.sat_t >= 0 && n > (unsigned) s.sat_t) {
             reason = kExitSaturated;
             break;
         }
         if (loop == kLoopDO) {
-            vsum += ld64_agent(&s.ctl->fedges[c.step % 3]);
-            int dir = choose_direction(s, c, n, vsum);
+            vsum += fe;
+            int dir = direction_rule(s, c.phase, fe, n, vsum);
+            if (start && dir != c.phase) {
+                grid_barrier(s.ctl);
+                start = false;
+            }
             if (dir == kPhaseBottomUp && c.phase != kPhaseBottomUp) {
                 frontier_bitmap_pass(g, s, c, n, 0);
                 grid_barrier(s.ctl);
@@ -988,8 +1083,11 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
             c.phase = dir;
         }
 
-        if (fast_level(s, c, n) && gridDim.x > 1 &&
-            ((LOOP != kLoopEC && LOOP != kLoopDO) || n <= (unsigned) kFastCap)) {
+        const bool fast = fast_level(s, c, n, fe) && gridDim.x > 1 &&
+                          ((LOOP != kLoopEC && LOOP != kLoopDO) || n <= (unsigned) kFastCap);
+        if (start && !fast) grid_barrier(s.ctl);
+        start = false;
+        if (fast) {
             // Single-workgroup fast path: block 0 runs consecutive small
             // levels with workgroup barriers only (no grid tier: it alone
             // works); the rest of the grid waits at the next grid barrier.
@@ -1068,6 +1166,7 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
         s.ctl->exit_step = c.step;
         s.ctl->exit_iteration = c.iteration;
         s.ctl->exit_phase = c.phase;
+        if (s.host_head) publish_status(s);
     }
     if (s.instrument && threadIdx.x == 0) {
         unsigned long long life = wall_clock64() - t_begin;
@@ -1131,8 +1230,24 @@ __global__ void reset_labels_kernel(int *depth, int *pred, unsigned *visited, in
 // every counter here, before the first step, replaces the reference's
 // iteration-0 counter reset inside every CTA (contract_expand_atomic/
 // kernel.cuh:307-332), which can race with CTA 0's first enqueue.
-__global__ void init_search_kernel(SearchArgs s, const int *row, int src, int label_source)
+// Starts a search: clears the control block, queues the source and, with
+// reset set, rewrites every label in the same pass (each element written by
+// exactly one thread, the source's with its start values), saving a launch.
+__global__ void init_search_kernel(SearchArgs s, const int *row, int src, int label_source, int reset)
 {
+    if (reset) {
+        const int words = (s.nodes + 31) >> 5;
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < s.nodes; i += gridDim.x * blockDim.x) {
+            const bool is_src = label_source && i == src;
+            s.depth[i] = is_src ? 0 : -1;
+            if (s.pred) s.pred[i] = is_src ? -2 : -1;
+            if (i < words) {
+                const bool holds = label_source && (src >> 5) == i;
+                s.visited[i] = holds ? 1u << (src & 31) : 0u;
+            }
+        }
+    }
+    if (blockIdx.x != 0) return;
     Control *ctl = s.ctl;
     unsigned *words = reinterpret_cast<unsigned *>(ctl);
     for (int i = threadIdx.x; i < (int) (sizeof(Control) / 4); i += blockDim.x) words[i] = 0u;
@@ -1143,9 +1258,11 @@ __global__ void init_search_kernel(SearchArgs s, const int *row, int src, int la
         s.q[0][0] = src;
         if (s.pq[0]) s.pq[0][0] = -2;
         if (label_source) {
-            s.depth[src] = 0;
-            if (s.pred) s.pred[src] = -2;
-            s.visited[src >> 5] |= 1u << (src & 31);
+            if (!reset) {
+                s.depth[src] = 0;
+                if (s.pred) s.pred[src] = -2;
+                s.visited[src >> 5] |= 1u << (src & 31);
+            }
             ctl->max_depth = 0;
             unsigned long long deg = (unsigned long long) (row[src + 1] - row[src]);
             ctl->fedges[0] = deg;
@@ -1247,9 +1364,12 @@ hipError_t launch_reset_labels(int *depth, int *pred, unsigned *visited, int nod
     return hipGetLastError();
 }
 
-hipError_t launch_init_search(const SearchArgs &s, const int *row, int src, int label_source, hipStream_t st)
+hipError_t launch_init_search(const SearchArgs &s, const int *row, int src, int label_source, bool reset,
+                              hipStream_t st)
 {
-    hipLaunchKernelGGL(init_search_kernel, dim3(1), dim3(256), 0, st, s, row, src, label_source);
+    int grid = 1;
+    if (reset) grid = std::min(16384, std::max(1, (s.nodes + 255) / 256));
+    hipLaunchKernelGGL(init_search_kernel, dim3(grid), dim3(256), 0, st, s, row, src, label_source, reset ? 1 : 0);
     return hipGetLastError();
 }
 
