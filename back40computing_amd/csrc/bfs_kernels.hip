@@ -299,13 +299,20 @@ __device__ void body_expand_contract(const GraphArgs &g, const SearchArgs &s, Cu
 // Bottom-up step of a direction-optimizing search (Beamer, Asanovic,
 // Patterson, "Direction-Optimizing Breadth-First Search", SC'12): every
 // unvisited vertex scans its row for a neighbour in the current frontier
-// bitmap fbm[d & 1] and stops at the first one.  Each wave owns 64
-// consecutive vertices, i.e. two whole bitmap words, so the visited and
-// next-frontier words are written with plain stores (the next-frontier word
-// is overwritten in full, which also clears it).  Short rows are scanned by
-// their own lane, four neighbours per round; rows of at least kBuLong
-// neighbours by the whole wave, 64 per round, with a ballot early exit.
+// bitmap fbm[d & 1] and stops at the first one (in row order).  Vertices go
+// in 64-vertex groups, one lane per vertex, so a group is two whole bitmap
+// words: the visited and next-frontier words are written with plain stores
+// (the next-frontier word is overwritten in full, which also clears it).
+//
+// The level is bound by each wave's chain of dependent loads (row bounds,
+// neighbours, frontier words), not by bytes, so a wave works on kBuGroups
+// groups at once: every lane scans its kBuGroups rows four neighbours per
+// round, with all rows' loads of a round in flight together.  Rows of at
+// least kBuLong neighbours get one such round (a high-degree vertex usually
+// has a frontier neighbour among its first few); the rest of a long row is
+// scanned by the whole wave, 64 neighbours per round, with a ballot exit.
 constexpr int kBuLong = 64;
+constexpr int kBuGroups = 2;
 
 template <bool PRED, bool EXACT>
 __device__ void body_bottom_up(const GraphArgs &g, const SearchArgs &s, Cursor c, Lds<PRED> &lds)
@@ -315,92 +322,132 @@ __device__ void body_bottom_up(const GraphArgs &g, const SearchArgs &s, Cursor c
     unsigned *nxt = s.fbm[(c.iteration + 1) & 1];
     const int lane = lane_id();
     const int next_depth = c.iteration + 1;
-    auto in_frontier = [&](int u) { return ((cur[u >> 5] >> (u & 31)) & 1u) != 0; };
     bool labeled = false;
     unsigned fe = 0;
     int found = 0;
-    // Static partition: every wave owns one contiguous run of 64-vertex
-    // groups (two whole bitmap words each) and walks it with no tile
-    // counter and no workgroup barrier, so the waves' dependent load chains
-    // overlap freely.  Every vertex costs about the same here, and vertex
-    // ids are scrambled, so the runs balance without stealing.
+    // Static partition: every wave owns one contiguous run of groups and
+    // walks it with no tile counter and no workgroup barrier, so the waves'
+    // load chains overlap freely.  Vertex ids are scrambled, so the runs
+    // balance without stealing.
     const unsigned groups = ((unsigned) g.nodes + kWave - 1) / kWave;
     const unsigned nwaves = gridDim.x * kWaves;
     const unsigned gw = blockIdx.x * kWaves + threadIdx.x / kWave;
     const unsigned per = (groups + nwaves - 1) / nwaves;
     const unsigned g_end = min(groups, (gw + 1) * per);
-    // Software pipeline: the visited word and row bounds of the next group
-    // are in flight while the current group scans, which takes one round
-    // trip off every group's dependent chain (row bounds are loaded for
-    // visited vertices too: they are contiguous and cheap).
-    unsigned gi = gw * per;
-    unsigned vis_n = ~0u;
-    int beg_n = 0, end_n = 0;
-    auto load_group = [&](unsigned gg) {
-        const int vv = (int) (gg * kWave + (unsigned) lane);
-        const bool ok = vv < g.nodes;
-        vis_n = ok ? s.visited[(unsigned) vv >> 5] : ~0u;
-        beg_n = ok ? g.rrow[vv] : 0;
-        end_n = ok ? g.rrow[vv + 1] : 0;
+    // Software pipeline: the visited words and row bounds of the next
+    // kBuGroups groups are in flight while the current ones scan (row bounds
+    // are loaded for visited vertices too: they are contiguous and cheap).
+    unsigned vis_n[kBuGroups];
+    int beg_n[kBuGroups], end_n[kBuGroups];
+    auto load_groups = [&](unsigned g0) {
+#pragma unroll
+        for (int k = 0; k < kBuGroups; k++) {
+            const int vv = (int) ((g0 + (unsigned) k) * kWave + (unsigned) lane);
+            const bool ok = g0 + (unsigned) k < g_end && vv < g.nodes;
+            vis_n[k] = ok ? s.visited[(unsigned) vv >> 5] : ~0u;
+            beg_n[k] = ok ? g.rrow[vv] : 0;
+            end_n[k] = ok ? g.rrow[vv + 1] : 0;
+        }
     };
-    if (gi < g_end) load_group(gi);
-    for (; gi < g_end; gi++) {
-        const unsigned vis = vis_n;
-        int beg = beg_n, end = end_n;
-        if (gi + 1 < g_end) load_group(gi + 1);
-        int v = (int) (gi * kWave + (unsigned) lane);
-        unsigned w = (unsigned) v >> 5;
-        bool valid = v < g.nodes;
-        bool todo = valid && ((vis >> (v & 31)) & 1u) == 0;
-        // Best-effort top-down levels may have lost visited bits to racing
-        // read-modify-writes; the label stays the authority there.
-        if (!EXACT && todo) todo = s.depth[v] == -1;
-        if (!todo) end = beg;
-        int len = end - beg;
-        int parent = -1;
-        if (todo && len < kBuLong) {
-            for (int e = beg; e < end; e += 4) {
-                int u0 = g.rcol[e];
-                int u1 = e + 1 < end ? g.rcol[e + 1] : -1;
-                int u2 = e + 2 < end ? g.rcol[e + 2] : -1;
-                int u3 = e + 3 < end ? g.rcol[e + 3] : -1;
-                if (in_frontier(u0)) { parent = u0; break; }
-                if (u1 >= 0 && in_frontier(u1)) { parent = u1; break; }
-                if (u2 >= 0 && in_frontier(u2)) { parent = u2; break; }
-                if (u3 >= 0 && in_frontier(u3)) { parent = u3; break; }
+    unsigned gi = gw * per;
+    if (gi < g_end) load_groups(gi);
+    for (; gi < g_end; gi += kBuGroups) {
+        unsigned vis[kBuGroups];
+        int beg[kBuGroups], end[kBuGroups], pos[kBuGroups], lim[kBuGroups], parent[kBuGroups];
+        bool todo[kBuGroups];
+#pragma unroll
+        for (int k = 0; k < kBuGroups; k++) {
+            vis[k] = vis_n[k];
+            beg[k] = beg_n[k];
+            end[k] = end_n[k];
+        }
+        if (gi + kBuGroups < g_end) load_groups(gi + kBuGroups);
+#pragma unroll
+        for (int k = 0; k < kBuGroups; k++) {
+            const int v = (int) ((gi + (unsigned) k) * kWave + (unsigned) lane);
+            todo[k] = gi + (unsigned) k < g_end && v < g.nodes && ((vis[k] >> (v & 31)) & 1u) == 0;
+        }
+        if (!EXACT) {
+            // Best-effort top-down levels may have lost visited bits to
+            // racing read-modify-writes; the label stays the authority.
+            int dep[kBuGroups];
+#pragma unroll
+            for (int k = 0; k < kBuGroups; k++)
+                dep[k] = todo[k] ? s.depth[(gi + (unsigned) k) * kWave + (unsigned) lane] : -1;
+#pragma unroll
+            for (int k = 0; k < kBuGroups; k++) todo[k] = todo[k] && dep[k] == -1;
+        }
+#pragma unroll
+        for (int k = 0; k < kBuGroups; k++) {
+            if (!todo[k]) end[k] = beg[k];
+            pos[k] = beg[k];
+            lim[k] = end[k] - beg[k] < kBuLong ? end[k] : min(end[k], beg[k] + 4);
+            parent[k] = -1;
+        }
+        // per-lane rounds of four neighbours of every row still searching
+        while (true) {
+            bool more = false;
+#pragma unroll
+            for (int k = 0; k < kBuGroups; k++) more |= parent[k] < 0 && pos[k] < lim[k];
+            if (!more) break;
+            int u[kBuGroups][4];
+#pragma unroll
+            for (int k = 0; k < kBuGroups; k++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    u[k][j] = parent[k] < 0 && pos[k] + j < lim[k] ? g.rcol[pos[k] + j] : -1;
+            unsigned fw[kBuGroups][4];
+#pragma unroll
+            for (int k = 0; k < kBuGroups; k++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) fw[k][j] = u[k][j] >= 0 ? cur[u[k][j] >> 5] : 0u;
+#pragma unroll
+            for (int k = 0; k < kBuGroups; k++) {
+#pragma unroll
+                for (int j = 3; j >= 0; j--)  // the first hit in row order wins
+                    if (u[k][j] >= 0 && ((fw[k][j] >> (u[k][j] & 31)) & 1u)) parent[k] = u[k][j];
+                pos[k] += 4;
             }
         }
-        uint64_t lm = ballot(todo && len >= kBuLong);
-        while (lm) {
-            int l = uniform(__builtin_ctzll(lm));
-            lm &= lm - 1;
-            int b = readlane(beg, l), e = readlane(end, l);
-            int hit = -1;
-            for (int k = b; k < e; k += kWave) {
-                int idx = k + lane;
-                int u = idx < e ? g.rcol[idx] : -1;
-                uint64_t hm = ballot(u >= 0 && in_frontier(u));
-                if (hm) {
-                    hit = readlane(u, uniform(__builtin_ctzll(hm)));
-                    break;
+        // the rest of the long rows, one row at a time by the whole wave
+#pragma unroll
+        for (int k = 0; k < kBuGroups; k++) {
+            uint64_t lm = ballot(parent[k] < 0 && lim[k] < end[k]);
+            while (lm) {
+                const int l = uniform(__builtin_ctzll(lm));
+                lm &= lm - 1;
+                const int b = readlane(lim[k], l), e = readlane(end[k], l);
+                int hit = -1;
+                for (int q = b; q < e; q += kWave) {
+                    const int idx = q + lane;
+                    const int uu = idx < e ? g.rcol[idx] : -1;
+                    const uint64_t hm = ballot(uu >= 0 && ((cur[uu >> 5] >> (uu & 31)) & 1u));
+                    if (hm) {
+                        hit = readlane(uu, uniform(__builtin_ctzll(hm)));
+                        break;
+                    }
                 }
+                if (lane == l) parent[k] = hit;
             }
-            if (lane == l) parent = hit;
         }
-        bool got = parent >= 0;
-        if (got) {
-            s.depth[v] = next_depth;
-            if (PRED) s.pred[v] = parent;
-            fe += (unsigned) len;
-            labeled = true;
+#pragma unroll
+        for (int k = 0; k < kBuGroups; k++) {
+            const unsigned v = (gi + (unsigned) k) * kWave + (unsigned) lane;
+            const bool got = parent[k] >= 0;
+            if (got) {
+                s.depth[v] = next_depth;
+                if (PRED) s.pred[v] = parent[k];
+                fe += (unsigned) (end[k] - beg[k]);
+                labeled = true;
+            }
+            const uint64_t gm = ballot(got);
+            if ((lane & 31) == 0 && gi + (unsigned) k < g_end && (int) v < g.nodes) {
+                const unsigned bits = (unsigned) (lane == 0 ? gm : gm >> 32);
+                if (bits) s.visited[v >> 5] = vis[k] | bits;
+                nxt[v >> 5] = bits;
+            }
+            found += got ? 1 : 0;
         }
-        uint64_t gm = ballot(got);
-        if ((lane & 31) == 0 && valid) {
-            unsigned bits = (unsigned) (lane == 0 ? gm : gm >> 32);
-            if (bits) s.visited[w] = vis | bits;
-            nxt[w] = bits;
-        }
-        found += got ? 1 : 0;
     }
     // The level's output is the bitmap nxt; only its size goes to the queue
     // length (queue_from_bitmap materialises the queue if the search turns
@@ -559,6 +606,26 @@ __device__ __forceinline__ void run_body(int phase, const GraphArgs &g, const Se
     case kPhaseExpand: body_expand<PRED, EXACT>(g, s, c, n, lds, sub, hubs); break;
     case kPhaseBottomUp: body_bottom_up<PRED, EXACT>(g, s, c, lds); break;
     default: body_filter<PRED, EXACT>(g, s, c, n, lds); break;
+    }
+}
+
+// The phases a persistent loop can run (the others are not compiled into
+// its kernel, which keeps their registers out of its budget).
+template <bool PRED, bool EXACT, int LOOP>
+__device__ __forceinline__ void loop_body(int phase, const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n,
+                                          Lds<PRED> &lds, int sub, bool hubs)
+{
+    if constexpr (LOOP == kLoopEC) {
+        body_expand_contract<PRED, EXACT>(g, s, c, n, lds, sub, hubs);
+    } else if constexpr (LOOP == kLoopCE) {
+        body_contract_expand<PRED, EXACT>(g, s, c, n, lds, sub, hubs);
+    } else if constexpr (LOOP == kLoopDO) {
+        if (phase == kPhaseBottomUp) body_bottom_up<PRED, EXACT>(g, s, c, lds);
+        else body_expand_contract<PRED, EXACT>(g, s, c, n, lds, sub, hubs);
+    } else {
+        if (phase == kPhaseContract) body_contract<PRED, EXACT>(g, s, c, n, lds);
+        else if (phase == kPhaseExpand) body_expand<PRED, EXACT>(g, s, c, n, lds, sub, hubs);
+        else body_filter<PRED, EXACT>(g, s, c, n, lds);
     }
 }
 
@@ -1112,7 +1179,7 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
                 unsigned long long fv = vsum;  // sum through the step being run
                 while (true) {
                     if (threadIdx.x == 0) prepare_step(s.ctl, c, n);
-                    run_body<PRED, EXACT>(c.phase, g, s, c, n, lds, 0, false);
+                    loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 0, false);
                     // the next level reads queue entries other waves just
                     // stored: each wave's stores must have completed, which
                     // __syncthreads() alone does not wait for
@@ -1149,12 +1216,12 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
 
         unsigned long long t0 = s.instrument ? wall_clock64() : 0;
         if (blockIdx.x == 0 && threadIdx.x == 0) prepare_step(s.ctl, c, n);
-        run_body<PRED, EXACT>(c.phase, g, s, c, n, lds, 0, true);
+        loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 0, true);
         if (s.instrument) busy += wall_clock64() - t0;
         grid_barrier(s.ctl);
         if (gathers(c.phase) && hubs_pending(s, c.step)) {
             t0 = s.instrument ? wall_clock64() : 0;
-            run_body<PRED, EXACT>(c.phase, g, s, c, n, lds, 1, true);
+            loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 1, true);
             if (s.instrument) busy += wall_clock64() - t0;
             grid_barrier(s.ctl);
         }
