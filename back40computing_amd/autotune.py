@@ -35,10 +35,11 @@ TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.jso
 
 # Values tried per parameter, in the order the descent visits them.
 SPACE_SINGLE = {
-    "blocks_per_cu": [1, 2, 4],
+    "blocks_per_cu": [1, 2],
     "tile_grab_max": [1, 4, 16, 32, 64],
     "wave_gather_threshold": [32, 64, 128],
-    "cta_gather_threshold": [64, 128, 256, 512, 1024],
+    # rows shorter than a 512-thread workgroup waste most of it in the CTA tier
+    "cta_gather_threshold": [256, 512, 1024],
     "grid_gather_threshold": [2048, 4096, 8192, 32768, -1],
     "do_alpha": [2, 4, 8, 14, 24],
     "do_beta": [8, 16, 24, 48],

@@ -49,10 +49,10 @@ bfs_status hip_fail(hipError_t e, const char *what)
 // back40computing_amd/tuned.json).
 const bfs_config kDefaults = {
     /*wave_gather_threshold*/ 64,
-    /*cta_gather_threshold*/ 64,
+    /*cta_gather_threshold*/ 512,
     /*fast_path_threshold*/ 1024,
     /*saturation_threshold*/ 0,   // 0: persistent grid size * 256
-    /*blocks_per_cu*/ 2,
+    /*blocks_per_cu*/ 1,
     /*cull_mode*/ BFS_CULL_EXACT,
     /*instrument*/ 0,
     /*grid_gather_threshold*/ 8192,

@@ -16,8 +16,6 @@
 namespace b40c_hip {
 
 constexpr int kWave = 64;
-constexpr int kBlock = 256;                    // threads per workgroup
-constexpr int kWaves = kBlock / kWave;         // waves per workgroup
 constexpr int kWaveQueueCap = 512;             // LDS staging entries per wave
 constexpr int kHistory = 1024;                 // LDS history-hash entries per workgroup
 constexpr int kBarrierFanIn = 16;              // first-level barrier counters
@@ -80,6 +78,25 @@ struct HubEntry {
 };
 
 enum ExitReason { kExitDone = 0, kExitSaturated = 1, kExitOverflow = 2 };
+
+// ---------------------------------------------------------------------------
+// Threads per workgroup, chosen per translation unit.  The single-source
+// kernels run 512: a grid barrier's release and acquire fences cost about
+// the same per workgroup, so fewer, larger workgroups halve its cost at the
+// same wave count.  The batch kernels build with 256 (B40C_BLOCK=256).
+// Everything below depends on it and sits in an inline namespace named
+// after the size, so the two builds never share a symbol.
+// ---------------------------------------------------------------------------
+#ifndef B40C_BLOCK
+#define B40C_BLOCK 512
+#endif
+#define B40C_NS_CAT2(a, b) a##b
+#define B40C_NS_CAT(a, b) B40C_NS_CAT2(a, b)
+inline namespace B40C_NS_CAT(block, B40C_BLOCK) {
+
+constexpr int kBlock = B40C_BLOCK;
+constexpr int kWaves = kBlock / kWave;         // waves per workgroup
+static_assert(kBlock % kWave == 0 && kBlock >= 128 && kBlock <= 1024, "workgroup of whole waves");
 
 // ---------------------------------------------------------------------------
 // Wave64 primitives
@@ -501,4 +518,5 @@ __device__ __forceinline__ void grid_barrier(Control *ctl)
     __syncthreads();
 }
 
+}  // inline namespace (workgroup size)
 }  // namespace b40c_hip

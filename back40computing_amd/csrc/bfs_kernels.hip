@@ -122,7 +122,9 @@ __device__ __forceinline__ HubList hub_list(const SearchArgs &s, int step, bool 
     h.cap = s.hub_cap;
     h.alloc = &s.ctl->hub_alloc[step & 1];
     h.grid_t = enabled ? s.grid_t : 0x7fffffff;
-    if (enabled && narrow_step(n)) h.grid_t = min(h.grid_t, s.cta_t);
+    // a step of fewer tiles than workgroups leaves most workgroups idle:
+    // every row long enough for the wave tier goes to the grid instead
+    if (enabled && narrow_step(n)) h.grid_t = min(h.grid_t, max(s.wave_t, kWave));
     h.tag = 0;
     h.chunk = hub_chunk(n);
     return h;
@@ -456,20 +458,27 @@ __device__ void body_bottom_up(const GraphArgs &g, const SearchArgs &s, Cursor c
 }
 
 // Writes the members of frontier bitmap fbm[c.iteration & 1] into the input
-// queue of step c (a bottom-up level records its output only as a bitmap).
+// queue of step c (a bottom-up level records its output only as a bitmap),
+// clearing both frontier bitmaps on the way.
 // Grid-wide; a grid barrier or kernel boundary must follow before the queue
 // is read.  Reservations come from ctl->compact, which the previous step
 // zeroed.
 template <bool PRED>
 __device__ void queue_from_bitmap(const GraphArgs &g, const SearchArgs &s, Cursor c, Lds<PRED> &lds)
 {
-    const unsigned *bm = s.fbm[c.iteration & 1];
+    unsigned *bm = s.fbm[c.iteration & 1];
+    unsigned *other = s.fbm[(c.iteration + 1) & 1];
     int *out = s.q[c.step & 1];
     const unsigned words = ((unsigned) g.nodes + 31) >> 5;
     const unsigned stride = gridDim.x * kBlock;
     for (unsigned w0 = blockIdx.x * kBlock; w0 < words; w0 += stride) {
         const unsigned w = w0 + threadIdx.x;
         const unsigned bits = w < words ? bm[w] : 0u;
+        // top-down levels leave both bitmaps clear (see frontier_bitmap_pass)
+        if (w < words) {
+            bm[w] = 0u;
+            other[w] = 0u;
+        }
         int total = 0;
         int excl = block_exclusive_sum(__popc(bits), lds.ep_scan, &total);
         if (threadIdx.x == 0) lds.ep_base = total ? add_agent(&s.ctl->compact, (unsigned) total) : 0u;
@@ -631,7 +640,10 @@ __device__ __forceinline__ void loop_body(int phase, const GraphArgs &g, const S
 
 // Two grid-wide passes that turn the vertex queue of step c into the
 // frontier bitmap fbm[c.iteration & 1] (zero, then set); a grid barrier or
-// kernel boundary must separate them.
+// kernel boundary must separate them.  A persistent search skips the first:
+// its prologue clears both bitmaps and queue_from_bitmap clears them again
+// whenever the search turns top-down, so they are clear at every top-down
+// level.
 __device__ __forceinline__ void frontier_bitmap_pass(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n,
                                                      int pass)
 {
@@ -747,6 +759,9 @@ __device__ __forceinline__ bool hubs_pending(const SearchArgs &s, int step)
 // ---------------------------------------------------------------------------
 constexpr int kFastCap = 1024;  // frontier members kept in LDS
 constexpr int kFastK = 4;       // consecutive edges per thread per batch
+// Most edges of a level run by the path: a batch of kBlock * kFastK edges
+// costs about one quarter of a grid-wide step, so a few batches still win.
+constexpr unsigned long long kFastEdges = 4ull * kBlock * kFastK;
 static_assert(kFastCap <= 4 * kBlock, "the degree scan takes four members per thread");
 
 struct FastFront {
@@ -975,8 +990,7 @@ __device__ void fast_lds_path(const GraphArgs &g, const SearchArgs &s, Cursor c,
         ve_add += fe2;
         src ^= 1;
         c = advance(loop, c);
-        bool go = n2 > 0 && n2 <= (unsigned) kFastCap && n2 <= (unsigned) s.fast_t &&
-                  fe2 <= 32ull * (unsigned) s.fast_t;
+        bool go = n2 > 0 && n2 <= (unsigned) kFastCap && n2 <= (unsigned) s.fast_t && fe2 <= kFastEdges;
         if (go && loop == kLoopDO) {
             const unsigned long long fv2 = fv + fe2;
             if (direction_rule(s, c.phase, fe2, n2, fv2) != c.phase) go = false;
@@ -1027,9 +1041,14 @@ __device__ void fast_lds_path(const GraphArgs &g, const SearchArgs &s, Cursor c,
 // before a grid barrier as long as it reads nothing of the control block
 // and claims nothing before one (the loop top takes step 0's values from
 // search_start_values).
-__device__ void search_prologue(const GraphArgs &g, const SearchArgs &s)
+__device__ void search_prologue(const GraphArgs &g, const SearchArgs &s, bool clear_bitmaps)
 {
     const int src = s.init_src;
+    if (clear_bitmaps) {
+        // both frontier bitmaps, which are contiguous
+        const int words = 2 * ((s.nodes + 31) >> 5);
+        for (int i = blockIdx.x * kBlock + threadIdx.x; i < words; i += gridDim.x * kBlock) s.fbm[0][i] = 0u;
+    }
     if (s.init_reset) {
         const int words = (s.nodes + 31) >> 5;
         for (int i = blockIdx.x * kBlock + threadIdx.x; i < s.nodes; i += gridDim.x * kBlock) {
@@ -1109,7 +1128,7 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
     // 0 is still writing, and the step's first grid barrier orders the
     // label rewrite before any claim.
     bool start = s.init_src >= 0;
-    if (start) search_prologue(g, s);
+    if (start) search_prologue(g, s, LOOP == kLoopDO);
 
     // Every decision at the top of a step reads only values final before
     // the step began and unchanged during it (the input length, the
@@ -1139,8 +1158,10 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
                 start = false;
             }
             if (dir == kPhaseBottomUp && c.phase != kPhaseBottomUp) {
-                frontier_bitmap_pass(g, s, c, n, 0);
-                grid_barrier(s.ctl);
+                if (s.init_src < 0) {  // bitmaps not known to be clear
+                    frontier_bitmap_pass(g, s, c, n, 0);
+                    grid_barrier(s.ctl);
+                }
                 frontier_bitmap_pass(g, s, c, n, 1);
                 grid_barrier(s.ctl);
             } else if (dir != kPhaseBottomUp && c.phase == kPhaseBottomUp) {
@@ -1151,7 +1172,7 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
         }
 
         const bool fast = fast_level(s, c, n, fe) && gridDim.x > 1 &&
-                          ((LOOP != kLoopEC && LOOP != kLoopDO) || n <= (unsigned) kFastCap);
+                          ((LOOP != kLoopEC && LOOP != kLoopDO) || (n <= (unsigned) kFastCap && fe <= kFastEdges));
         if (start && !fast) grid_barrier(s.ctl);
         start = false;
         if (fast) {
