@@ -46,7 +46,7 @@ struct PoolGraph {
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-constexpr unsigned kBatchHubCapacity = 1u << 16;  // grid-tier rows per level
+constexpr unsigned kBatchHubCapacity = 1u << 18;  // grid-tier rows per level
 
 // Per-query device state layout inside an arena (256-byte aligned pieces).
 struct SlotLayout {

@@ -44,7 +44,8 @@ struct alignas(256) BatchControl {
     unsigned bar_pad[26];
     unsigned long long hub_alloc[2];  // grid tier: (entries << 32) | chunks, by level & 1
     unsigned hub_steal[2];            // grid tier: chunk counter, by level & 1
-    unsigned pad1[28];
+    int hub_chunk[2];                 // grid tier: edges per chunk, by level & 1
+    unsigned pad1[26];
     // grid barrier state (same layout discipline as Control)
     Control bar;
     // per-level record: wall clock at the level start, top-down and

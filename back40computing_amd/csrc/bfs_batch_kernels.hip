@@ -31,6 +31,9 @@ struct BatchLds {
     // through the step being planned (vtop)
     unsigned long long vsum[kMaxSlots];
     unsigned long long vtop[kMaxSlots];
+    int grid_t;              // grid-tier row threshold of this level
+    int chunk;               // grid-tier chunk of this level
+    int bu_tile;             // kBlock-vertex blocks per bottom-up tile of this level
 };
 
 __device__ __forceinline__ unsigned next_item_batch(unsigned *counter, Lds<false> &lds, unsigned k = 1u)
@@ -87,8 +90,8 @@ __device__ int batch_plan(const BatchArgs &a, BatchLds &L, int level)
             L.vtop[tid] = L.vsum[tid] + ld64(&sl.fedges[step % 3]);
             if (n > 0) {
                 dir = batch_direction(a, sl, g, L.phase[tid], step, (unsigned) n, L.vtop[tid]);
-                const int bu = kBlock * a.bu_tile;
-                tiles = dir == kPhaseBottomUp ? (g.nodes + bu - 1) / bu : (n + kBlock - 1) / kBlock;
+                // bottom-up: kBlock-vertex blocks here, grouped into tiles below
+                tiles = dir == kPhaseBottomUp ? (g.nodes + kBlock - 1) / kBlock : (n + kBlock - 1) / kBlock;
                 if (blockIdx.x == 0) {
                     st_agent(&sl.qlen[(step + 2) % 3], 0u);
                     st_agent(&sl.ovf[(step + 2) % 3], 0u);
@@ -114,12 +117,28 @@ __device__ int batch_plan(const BatchArgs &a, BatchLds &L, int level)
         st_agent(&a.ctl->hub_steal[(level + 1) & 1], 0u);
         __hip_atomic_store(&a.ctl->hub_alloc[(level + 1) & 1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    // Bottom-up tile size of this level: up to bu_tile blocks, fewer when
+    // the level has too few blocks to give every workgroup about eight
+    // tiles (one tile of a small graph would otherwise hold a whole
+    // level up while the grid waits at the barrier).
+    int bu_blocks = 0;
+    (void) block_exclusive_sum(tid < a.nslots && L.dir[tid] == kPhaseBottomUp ? tiles : 0, L.wave_tot, &bu_blocks);
+    const int bu_tile = max(1, min(a.bu_tile, bu_blocks / (8 * (int) gridDim.x)));
+    if (tid < a.nslots && L.dir[tid] == kPhaseBottomUp) tiles = (tiles + bu_tile - 1) / bu_tile;
+    if (tid == 0) L.bu_tile = bu_tile;
     int total = 0;
     int bu_total = 0;
     (void) block_exclusive_sum(tid < a.nslots && L.dir[tid] == kPhaseBottomUp ? tiles : 0, L.wave_tot, &bu_total);
     int excl = block_exclusive_sum(tiles, L.wave_tot, &total);
     if (tid < a.nslots) L.prefix[tid] = excl;
     if (tid == 0) L.prefix[a.nslots] = total;
+    // A level of fewer tiles than workgroups leaves most of them idle while
+    // a few gather long rows alone: every row long enough for the wave tier
+    // then goes to the grid tier, in workgroup-sized chunks.
+    const bool narrow = total < (int) gridDim.x;
+    L.grid_t = narrow ? min(a.grid_t, max(a.wave_t, kWave)) : a.grid_t;
+    L.chunk = narrow ? kBlock : kHubChunk;
+    if (blockIdx.x == 0 && tid == 0) a.ctl->hub_chunk[level & 1] = L.chunk;
     if (blockIdx.x == 0 && tid == 0 && level < kLevelLog) {
         a.ctl->level_clock[level] = wall_clock64();
         a.ctl->level_td_tiles[level] = (unsigned) (total - bu_total);
@@ -207,7 +226,7 @@ __device__ void batch_td_tile(const BatchArgs &a, BatchLds &L, int q, unsigned l
     const BatchGraph g = a.graphs[sl.graph];
     const int step = L.step[q];
     const unsigned n = (unsigned) L.n[q];
-    HubList hub{a.hubs[level & 1], a.hub_cap, &a.ctl->hub_alloc[level & 1], a.grid_t, q, kHubChunk};
+    HubList hub{a.hubs[level & 1], a.hub_cap, &a.ctl->hub_alloc[level & 1], L.grid_t, q, L.chunk};
     unsigned i = local * kBlock + threadIdx.x;
     int v = -1, beg = 0, end = 0;
     if (i < n) {
@@ -218,7 +237,7 @@ __device__ void batch_td_tile(const BatchArgs &a, BatchLds &L, int q, unsigned l
     gather_rows<false>(beg, end, v, g.col, a.wave_t, a.cta_t, hub, L.base, TdEdge{sl, g, L.it[q] + 1, o, a.coherent_filter != 0});
 }
 
-// Grid-tier phase of a level: every workgroup takes kHubChunk-edge chunks of
+// Grid-tier phase of a level: every workgroup takes chunks of
 // the rows the top-down tiles published.  Follows a grid barrier (or kernel
 // boundary) after the tiles.
 __device__ void batch_hub_phase(const BatchArgs &a, BatchLds &L, int level)
@@ -229,7 +248,9 @@ __device__ void batch_hub_phase(const BatchArgs &a, BatchLds &L, int level)
     if (entries == 0) return;
     const HubEntry *list = a.hubs[level & 1];
     HubEntry last = list[entries - 1];
-    unsigned total = last.first_chunk + (unsigned) ((last.end - last.beg + kHubChunk - 1) / kHubChunk);
+    // the chunk the level's tiles published with (set when it was planned)
+    const int chunk = ld_agent(&a.ctl->hub_chunk[level & 1]);
+    unsigned total = last.first_chunk + (unsigned) ((last.end - last.beg + chunk - 1) / chunk);
     SlotOut o;
     o.q = -1;
     // strided grabs of chunks: one counter atomic serves up to grab_max chunks
@@ -261,9 +282,9 @@ __device__ void batch_hub_phase(const BatchArgs &a, BatchLds &L, int level)
                 else hi = mid - 1;
             }
             HubEntry e = list[lo];
-            int b = e.beg + (int) (ch - e.first_chunk) * kHubChunk;
+            int b = e.beg + (int) (ch - e.first_chunk) * chunk;
             L.base.cta_beg = b;
-            L.base.cta_end = min(e.end, b + kHubChunk);
+            L.base.cta_end = min(e.end, b + chunk);
             L.base.cta_par = e.par;
             L.base.iteration = e.tag;
         }
@@ -288,93 +309,144 @@ __device__ __forceinline__ bool batch_hubs_pending(const BatchArgs &a, int level
     return ld64(&a.ctl->hub_alloc[level & 1]) != 0ull;
 }
 
-// One kBlock-vertex block of a bottom-up tile of query q.  The
-// frontier test is depth[u] == current depth after a top-down level (a vertex
-// labelled during this level holds depth + 1, so it never passes) and the
-// frontier bitmap after a bottom-up level; each wave owns two words of the
-// visited and next-frontier bitmaps.
-__device__ void batch_bu_block(const BatchArgs &a, BatchLds &L, int q, unsigned local, SlotOut &o)
+// Bottom-up tile of query q: L.bu_tile consecutive kBlock-vertex blocks, each
+// lane owning one vertex of every block (each wave two words of the visited
+// and next-frontier bitmaps per block).  The frontier test is the bitmap
+// written by a bottom-up previous level, or depth[u] == current depth after
+// a top-down one (a vertex labelled during this level holds depth + 1, or
+// still -1 if its store is not yet visible, and fails either way).
+//
+// As in the single-source bottom-up step, the level is bound by dependent
+// load chains: a wave scans kBatchBuGroups blocks at once, four neighbours
+// of every row per round with all loads of a round in flight (the visited
+// word and the label of a neighbour are read together), long rows get one
+// such round before the wave-wide scan of the rest.
+constexpr int kBatchBuGroups = 1;
+constexpr int kBatchBuLong = 64;
+
+__device__ void batch_bu_tile(const BatchArgs &a, BatchLds &L, int q, unsigned local, SlotOut &o)
 {
     BatchSlot &sl = a.slots[q];
     const BatchGraph g = a.graphs[sl.graph];
     const int d = L.it[q];
     const int lane = lane_id();
     WaveQueue<false> &wq = o.wq;
-    unsigned &fe = o.fe;
-    // After a bottom-up level the current frontier is the bitmap that level
-    // wrote (L2-resident); after a top-down level it is read from the labels.
     const bool use_bitmap = L.phase[q] == kPhaseBottomUp;
     const unsigned *cur = sl.fbm[d & 1];
     unsigned *nxt = sl.fbm[(d + 1) & 1];
-    // After a top-down level the visited bitmap (small, cache-resident) is
-    // tested first, so only visited neighbours cost a label read; a vertex
-    // labelled during this level holds depth d + 1 (or still -1 if its
-    // label store is not yet visible) and fails the label test either way.
-    auto in_frontier = [&](int u) {
-        if (use_bitmap) return ((cur[u >> 5] >> (u & 31)) & 1u) != 0;
-        return ((sl.visited[u >> 5] >> (u & 31)) & 1u) != 0 && sl.depth[u] == d;
-    };
-    int v = (int) (local * kBlock + threadIdx.x);
-    unsigned w = (unsigned) v >> 5;
-    bool valid = v < g.nodes;
-    unsigned vis = valid ? sl.visited[w] : ~0u;
-    bool todo = valid && ((vis >> (v & 31)) & 1u) == 0;
-    int beg = 0, end = 0;
-    if (todo) {
-        beg = g.rrow[v];
-        end = g.rrow[v + 1];
-    }
-    int len = end - beg, parent = -1;
-    if (todo && len < 64) {
-        for (int e = beg; e < end; e += 4) {
-            int u0 = g.rcol[e];
-            int u1 = e + 1 < end ? g.rcol[e + 1] : -1;
-            int u2 = e + 2 < end ? g.rcol[e + 2] : -1;
-            int u3 = e + 3 < end ? g.rcol[e + 3] : -1;
-            if (in_frontier(u0)) { parent = u0; break; }
-            if (u1 >= 0 && in_frontier(u1)) { parent = u1; break; }
-            if (u2 >= 0 && in_frontier(u2)) { parent = u2; break; }
-            if (u3 >= 0 && in_frontier(u3)) { parent = u3; break; }
+    const unsigned nblocks = (unsigned) (g.nodes + kBlock - 1) / kBlock;
+    const unsigned first = local * (unsigned) L.bu_tile;
+    const unsigned last = min(nblocks, first + (unsigned) L.bu_tile);
+    constexpr int G = kBatchBuGroups;
+    // software pipeline: the next blocks' visited words and row bounds are
+    // in flight while the current ones scan
+    unsigned vis_n[G];
+    int beg_n[G], end_n[G];
+    auto load_blocks = [&](unsigned b) {
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+            const int v = (int) ((b + (unsigned) k) * kBlock + threadIdx.x);
+            const bool ok = b + (unsigned) k < last && v < g.nodes;
+            vis_n[k] = ok ? sl.visited[(unsigned) v >> 5] : ~0u;
+            beg_n[k] = ok ? g.rrow[v] : 0;
+            end_n[k] = ok ? g.rrow[v + 1] : 0;
         }
-    }
-    uint64_t lm = ballot(todo && len >= 64);
-    while (lm) {
-        int l = uniform(__builtin_ctzll(lm));
-        lm &= lm - 1;
-        int b = readlane(beg, l), e = readlane(end, l);
-        int found = -1;
-        for (int k = b; k < e; k += kWave) {
-            int idx = k + lane;
-            int u = idx < e ? g.rcol[idx] : -1;
-            uint64_t hm = ballot(u >= 0 && in_frontier(u));
-            if (hm) {
-                found = readlane(u, uniform(__builtin_ctzll(hm)));
-                break;
+    };
+    if (first < last) load_blocks(first);
+    for (unsigned b0 = first; b0 < last; b0 += G) {
+        unsigned vis[G];
+        int beg[G], end[G], pos[G], lim[G], parent[G];
+        bool todo[G], valid[G];
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+            const int v = (int) ((b0 + (unsigned) k) * kBlock + threadIdx.x);
+            valid[k] = b0 + (unsigned) k < last && v < g.nodes;
+            vis[k] = vis_n[k];
+            beg[k] = beg_n[k];
+            end[k] = end_n[k];
+        }
+        if (b0 + G < last) load_blocks(b0 + G);
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+            const int v = (int) ((b0 + (unsigned) k) * kBlock + threadIdx.x);
+            todo[k] = valid[k] && ((vis[k] >> (v & 31)) & 1u) == 0;
+            if (!todo[k]) end[k] = beg[k];
+            pos[k] = beg[k];
+            lim[k] = end[k] - beg[k] < kBatchBuLong ? end[k] : min(end[k], beg[k] + 4);
+            parent[k] = -1;
+        }
+        while (true) {
+            bool more = false;
+#pragma unroll
+            for (int k = 0; k < G; k++) more |= parent[k] < 0 && pos[k] < lim[k];
+            if (!more) break;
+            int u[G][4];
+#pragma unroll
+            for (int k = 0; k < G; k++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    u[k][j] = parent[k] < 0 && pos[k] + j < lim[k] ? g.rcol[pos[k] + j] : -1;
+            unsigned fw[G][4];
+            int lab[G][4];
+#pragma unroll
+            for (int k = 0; k < G; k++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int uu = u[k][j];
+                    fw[k][j] = uu >= 0 ? (use_bitmap ? cur[uu >> 5] : sl.visited[uu >> 5]) : 0u;
+                    lab[k][j] = uu >= 0 && !use_bitmap ? sl.depth[uu] : d;
+                }
+#pragma unroll
+            for (int k = 0; k < G; k++) {
+#pragma unroll
+                for (int j = 3; j >= 0; j--)  // the first hit in row order wins
+                    if (u[k][j] >= 0 && ((fw[k][j] >> (u[k][j] & 31)) & 1u) && lab[k][j] == d) parent[k] = u[k][j];
+                pos[k] += 4;
             }
         }
-        if (lane == l) parent = found;
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+            uint64_t lm = ballot(parent[k] < 0 && lim[k] < end[k]);
+            while (lm) {
+                const int l = uniform(__builtin_ctzll(lm));
+                lm &= lm - 1;
+                const int bb = readlane(lim[k], l), e = readlane(end[k], l);
+                int hit = -1;
+                for (int r = bb; r < e; r += kWave) {
+                    const int idx = r + lane;
+                    const int uu = idx < e ? g.rcol[idx] : -1;
+                    bool in = false;
+                    if (uu >= 0) {
+                        const unsigned w = use_bitmap ? cur[uu >> 5] : sl.visited[uu >> 5];
+                        const int dl = use_bitmap ? d : sl.depth[uu];
+                        in = ((w >> (uu & 31)) & 1u) && dl == d;
+                    }
+                    const uint64_t hm = ballot(in);
+                    if (hm) {
+                        hit = readlane(uu, uniform(__builtin_ctzll(hm)));
+                        break;
+                    }
+                }
+                if (lane == l) parent[k] = hit;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < G; k++) {
+            const int v = (int) ((b0 + (unsigned) k) * kBlock + threadIdx.x);
+            const bool got = parent[k] >= 0;
+            if (got) {
+                sl.depth[v] = d + 1;
+                o.fe += (unsigned) (g.row[v + 1] - g.row[v]);
+            }
+            const uint64_t gm = ballot(got);
+            if ((lane & 31) == 0 && valid[k]) {
+                const unsigned bits = (unsigned) (lane == 0 ? gm : gm >> 32);
+                if (bits) sl.visited[(unsigned) v >> 5] = vis[k] | bits;
+                nxt[(unsigned) v >> 5] = bits;  // whole word: also clears it
+            }
+            wq.push(got, v, parent[k]);
+        }
     }
-    bool got = parent >= 0;
-    if (got) {
-        sl.depth[v] = d + 1;
-        fe += (unsigned) (g.row[v + 1] - g.row[v]);
-    }
-    uint64_t gm = ballot(got);
-    if ((lane & 31) == 0 && valid) {
-        unsigned bits = (unsigned) (lane == 0 ? gm : gm >> 32);
-        if (bits) sl.visited[w] = vis | bits;
-        nxt[w] = bits;  // whole word: also clears it
-    }
-    wq.push(got, v, parent);
-}
-
-// One bottom-up tile: bu_tile consecutive kBlock-vertex blocks.
-__device__ void batch_bu_tile(const BatchArgs &a, BatchLds &L, int q, unsigned local, SlotOut &o)
-{
-    const unsigned nblocks = (unsigned) (a.graphs[a.slots[q].graph].nodes + kBlock - 1) / kBlock;
-    const unsigned first = local * (unsigned) a.bu_tile;
-    const unsigned last = min(nblocks, first + (unsigned) a.bu_tile);
-    for (unsigned b = first; b < last; b++) batch_bu_block(a, L, q, b, o);
 }
 
 // Runs the tiles of one planned level.
