@@ -48,12 +48,13 @@ SPACE_SINGLE = {
 SPACE_BATCH = {
     "blocks_per_cu": [1, 2, 4],
     "tile_grab_max": [1, 8, 32, 128],
-    "bu_tile_blocks": [4, 8, 16, 32, 64],
+    # an upper bound: each level shrinks it to give every workgroup tiles
+    "bu_tile_blocks": [16, 64, 256],
     "grab_strided": [1, 2],
     "wave_gather_threshold": [32, 64, 128],
-    "cta_gather_threshold": [512, 1024, 2048],
-    "grid_gather_threshold": [4096, 8192, 32768],
-    "do_alpha": [2, 4, 8, 14],
+    "cta_gather_threshold": [256, 512, 1024],
+    "grid_gather_threshold": [512, 1024, 4096],
+    "do_alpha": [2, 4, 8],
     "do_beta": [8, 24, 48],
 }
 

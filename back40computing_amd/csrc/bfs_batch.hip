@@ -265,17 +265,18 @@ bfs_status bfs_batch_config_default(bfs_config *cfg)
     if (!cfg) return host_fail(BFS_ERR_INVALID_ARG, "null config");
     bfs_config_default(cfg);
     // Autotuned on the 256-query R-MAT pool (64 graphs, scale 14..20):
-    // bottom-up tiles of 64 blocks (16384 consecutive vertices per
-    // workgroup), four workgroups per CU, large grabs, an earlier grid tier
-    // and a later switch to bottom-up than the single-source default.
+    // bottom-up tiles of up to 64 blocks (16384 consecutive vertices per
+    // workgroup; levels with few blocks use smaller ones), four workgroups
+    // per CU (capped at half the occupancy), large consecutive grabs and an
+    // early grid tier (rows of 512 neighbours and more).
     cfg->blocks_per_cu = 4;
     cfg->bu_tile_blocks = 64;
-    cfg->tile_grab_max = 32;
-    cfg->grab_strided = 2;
-    cfg->grid_gather_threshold = 4096;
+    cfg->tile_grab_max = 128;
+    cfg->grab_strided = 1;
+    cfg->grid_gather_threshold = 512;
     cfg->wave_gather_threshold = 64;
     cfg->cta_gather_threshold = 1024;
-    cfg->do_alpha = 2;
+    cfg->do_alpha = 8;
     cfg->do_beta = 48;
     return BFS_OK;
 }
