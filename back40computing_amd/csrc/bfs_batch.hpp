@@ -33,6 +33,7 @@ struct alignas(128) BatchSlot {
     unsigned long long visited_edges;
     unsigned long long t_begin, t_end;  // wall clock at seeding / completion
     unsigned ovf[3];          // per-step overflow of the output queue, by step % 3
+    int bits;                 // fbm[iteration & 1] holds exactly the current frontier
     unsigned long long vbase; // sum of fedges over the steps before the current one
 };
 

@@ -24,13 +24,14 @@ struct BatchLds {
     int step[kMaxSlots];     // cursor copies
     int it[kMaxSlots];
     int phase[kMaxSlots];
-    int alive[kMaxSlots];
+    unsigned char alive[kMaxSlots];
     int prefix[kMaxSlots + 1];
     int wave_tot[kWaves];
     // visited-edge sums per query: through the previous step (vsum) and
     // through the step being planned (vtop)
     unsigned long long vsum[kMaxSlots];
     unsigned long long vtop[kMaxSlots];
+    unsigned char bits[kMaxSlots];  // the current frontier bitmap is exact (BatchSlot::bits)
     int grid_t;              // grid-tier row threshold of this level
     int chunk;               // grid-tier chunk of this level
     int bu_tile;             // kBlock-vertex blocks per bottom-up tile of this level
@@ -207,16 +208,32 @@ struct TdEdge {
     int next_depth;
     SlotOut &o;
     bool coherent;
+    unsigned *bits;  // next frontier's bitmap to mark claims in, or null
     __device__ __forceinline__ void operator()(bool ok, int u, int par) const
     {
         bool fresh = ok && claim_exact(sl.visited, u, coherent);
         if (fresh) {
             sl.depth[u] = next_depth;
             o.fe += (unsigned) (g.row[u + 1] - g.row[u]);
+            if (bits) __hip_atomic_fetch_or(bits + (u >> 5), 1u << (u & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         o.wq.push(fresh, u, par);
     }
 };
+
+// Frontier bitmaps across top-down levels.  A bottom-up level reads the
+// current frontier from fbm[d & 1] when it is exact (BatchSlot::bits) and
+// otherwise tests labels (depth == d), which costs a random label read per
+// neighbour.  A top-down level d marks its claims in fbm[(d + 1) & 1] when
+// that bitmap is clear -- when level d - 1 was top-down too, since every
+// top-down level clears fbm[d & 1], which nothing reads during it -- so
+// the usual top-down, top-down, bottom-up sequence reaches its first
+// bottom-up level with an exact bitmap.  A bottom-up level writes its whole
+// next-frontier bitmap, so it is always exact.
+__device__ __forceinline__ unsigned *td_bits(const BatchLds &L, BatchSlot &sl, int q)
+{
+    return L.phase[q] != kPhaseBottomUp ? sl.fbm[(L.it[q] + 1) & 1] : nullptr;
+}
 
 // One top-down (expand-contract) tile of query q.  Rows of at least grid_t
 // neighbours are published to the batch-wide grid-tier list (tagged with q).
@@ -234,7 +251,17 @@ __device__ void batch_td_tile(const BatchArgs &a, BatchLds &L, int q, unsigned l
         beg = g.row[v];
         end = g.row[v + 1];
     }
-    gather_rows<false>(beg, end, v, g.col, a.wave_t, a.cta_t, hub, L.base, TdEdge{sl, g, L.it[q] + 1, o, a.coherent_filter != 0});
+    {
+        // this tile's share of clearing fbm[d & 1] (see td_bits)
+        const unsigned words = ((unsigned) g.nodes + 31) >> 5;
+        const unsigned ntiles = (n + kBlock - 1) / kBlock;
+        const unsigned share = (words + ntiles - 1) / ntiles;
+        unsigned *old = sl.fbm[L.it[q] & 1];
+        const unsigned w_end = min(words, (local + 1) * share);
+        for (unsigned w = local * share + threadIdx.x; w < w_end; w += kBlock) old[w] = 0u;
+    }
+    gather_rows<false>(beg, end, v, g.col, a.wave_t, a.cta_t, hub, L.base,
+                       TdEdge{sl, g, L.it[q] + 1, o, a.coherent_filter != 0, td_bits(L, sl, q)});
 }
 
 // Grid-tier phase of a level: every workgroup takes chunks of
@@ -293,7 +320,7 @@ __device__ void batch_hub_phase(const BatchArgs &a, BatchLds &L, int level)
         out_open(o, a, L, q);
         BatchSlot &sl = a.slots[q];
         const BatchGraph g = a.graphs[sl.graph];
-        TdEdge edge{sl, g, L.it[q] + 1, o, a.coherent_filter != 0};
+        TdEdge edge{sl, g, L.it[q] + 1, o, a.coherent_filter != 0, td_bits(L, sl, q)};
         for (int k = b; k < e; k += kBlock) {
             int idx = k + threadIdx.x;
             bool ok = idx < e;
@@ -331,7 +358,7 @@ __device__ void batch_bu_tile(const BatchArgs &a, BatchLds &L, int q, unsigned l
     const int d = L.it[q];
     const int lane = lane_id();
     WaveQueue<false> &wq = o.wq;
-    const bool use_bitmap = L.phase[q] == kPhaseBottomUp;
+    const bool use_bitmap = L.bits[q] != 0;
     const unsigned *cur = sl.fbm[d & 1];
     unsigned *nxt = sl.fbm[(d + 1) & 1];
     const unsigned nblocks = (unsigned) (g.nodes + kBlock - 1) / kBlock;
@@ -503,6 +530,7 @@ __device__ void batch_load_cursors(const BatchArgs &a, BatchLds &L)
         L.phase[tid] = ld_agent(&sl.phase);
         L.alive[tid] = ld_agent(&sl.active) && !ld_agent(&sl.done);
         L.vsum[tid] = ld64(&sl.vbase);
+        L.bits[tid] = ld_agent(&sl.bits);
     }
     __syncthreads();
 }
@@ -527,6 +555,7 @@ __global__ __launch_bounds__(kBlock) void batch_persistent_kernel(BatchArgs a)
         if (tid < a.nslots && L.n[tid] > 0) {
             L.step[tid]++;
             L.it[tid]++;
+            L.bits[tid] = L.dir[tid] == kPhaseBottomUp || L.phase[tid] != kPhaseBottomUp;
             L.phase[tid] = L.dir[tid];
             L.vsum[tid] = L.vtop[tid];
         }
@@ -541,6 +570,7 @@ __global__ __launch_bounds__(kBlock) void batch_persistent_kernel(BatchArgs a)
             sl.iteration = L.it[tid];
             sl.phase = L.phase[tid];
             sl.vbase = L.vsum[tid];
+            sl.bits = L.bits[tid];
         }
         if (tid == 0) a.ctl->level = level;
     }
@@ -576,6 +606,7 @@ __global__ __launch_bounds__(kBlock) void batch_advance_kernel(BatchArgs a, int 
     if (tid < a.nslots) {
         BatchSlot &sl = a.slots[tid];
         if (sl.ran) {
+            sl.bits = sl.next_phase == kPhaseBottomUp || sl.phase != kPhaseBottomUp;
             sl.phase = sl.next_phase;
             sl.vbase += sl.fedges[sl.step % 3];
             sl.step++;
@@ -593,7 +624,7 @@ __global__ __launch_bounds__(kBlock) void batch_advance_kernel(BatchArgs a, int 
     }
 }
 
-// Resets depth / visited of the queries listed in `list` (blockIdx.y picks
+// Resets depth, visited and frontier bitmaps of the queries listed in `list` (blockIdx.y picks
 // the query).
 __global__ void batch_reset_kernel(BatchArgs a, const int *list)
 {
@@ -602,7 +633,11 @@ __global__ void batch_reset_kernel(BatchArgs a, const int *list)
     const int words = (nodes + 31) >> 5;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nodes; i += gridDim.x * blockDim.x) {
         sl.depth[i] = -1;
-        if (i < words) sl.visited[i] = 0u;
+        if (i < words) {
+            sl.visited[i] = 0u;
+            sl.fbm[0][i] = 0u;  // top-down levels mark claims in clear bitmaps
+            sl.fbm[1][i] = 0u;
+        }
     }
 }
 
@@ -631,6 +666,7 @@ __global__ void batch_seed_kernel(BatchArgs a, const int *list, int count)
     sl.overflow = 0;
     sl.ovf[0] = sl.ovf[1] = sl.ovf[2] = 0;
     sl.vbase = 0;
+    sl.bits = 0;  // fbm[0] does not hold the source: a bottom-up level 0 tests labels
     sl.ran = 0;
     sl.done = 0;
     sl.t_end = 0;
