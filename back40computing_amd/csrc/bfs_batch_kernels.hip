@@ -284,6 +284,7 @@ __device__ void batch_hub_phase(const BatchArgs &a, BatchLds &L, int level)
     const unsigned W = gridDim.x;
     const unsigned grab = max(1u, min((unsigned) a.grab_max, total / (W * 8u)));
     unsigned k_left = 0, ch_next = 0;
+    unsigned hint = 0;  // thread 0: entry of this workgroup's previous chunk
     while (true) {
         if (k_left == 0) {
             const unsigned i = next_item_batch(&a.ctl->hub_steal[level & 1], L.base);
@@ -302,12 +303,8 @@ __device__ void batch_hub_phase(const BatchArgs &a, BatchLds &L, int level)
             continue;
         }
         if (threadIdx.x == 0) {
-            unsigned lo = 0, hi = entries - 1;
-            while (lo < hi) {
-                unsigned mid = (lo + hi + 1) >> 1;
-                if (list[mid].first_chunk <= ch) lo = mid;
-                else hi = mid - 1;
-            }
+            const unsigned lo = hub_entry_of(list, entries, ch, hint);
+            hint = lo;
             HubEntry e = list[lo];
             int b = e.beg + (int) (ch - e.first_chunk) * chunk;
             L.base.cta_beg = b;

@@ -97,6 +97,7 @@ struct bfs_problem {
     Control *h_ctl_dev = nullptr;  // its device-side address
     HubEntry *d_hubs = nullptr;  // two grid-tier lists of kHubCapacity entries
     unsigned *d_fbm = nullptr;   // two frontier bitmaps (bottom-up levels)
+    unsigned char *d_marks = nullptr;  // discovery marks of large top-down levels (32 per visited word)
     int wall_khz = 100000;
     bfs_config cfg = kDefaults;
 };
@@ -155,6 +156,7 @@ SearchArgs search_args(const bfs_problem *p, int grid)
     size_t words = ((size_t) p->nodes + 31) / 32;
     s.fbm[0] = p->d_fbm;
     s.fbm[1] = p->d_fbm + words;
+    s.marks = p->d_marks;
     s.nodes = p->nodes;
     s.total_edges = (unsigned long long) p->edges;
     s.track_edges = 1;
@@ -444,8 +446,11 @@ bfs_status bfs_problem_create(const int32_t *h_row_offsets, const int32_t *h_col
         (e = hipMalloc(&p->d_visited, sizeof(unsigned) * words)) != hipSuccess ||
         (e = hipMalloc(&p->d_ctl, sizeof(Control))) != hipSuccess ||
         (e = hipMalloc(&p->d_hubs, sizeof(HubEntry) * 2 * kHubCapacity)) != hipSuccess ||
-        (e = hipMalloc(&p->d_fbm, sizeof(unsigned) * 2 * words)) != hipSuccess)
+        (e = hipMalloc(&p->d_fbm, sizeof(unsigned) * 2 * words)) != hipSuccess ||
+        (e = hipMalloc(&p->d_marks, 32 * words)) != hipSuccess)
         return cleanup(hip_fail(e, "allocate problem"));
+    // marks are clear between steps (each compaction clears what it reads)
+    if ((e = hipMemset(p->d_marks, 0, 32 * words)) != hipSuccess) return cleanup(hip_fail(e, "clear marks"));
     if (p->mark_pred && (e = hipMalloc(&p->d_pred, sizeof(int) * (size_t) nodes)) != hipSuccess)
         return cleanup(hip_fail(e, "allocate predecessors"));
     if ((e = hipHostMalloc(&p->h_ctl, sizeof(Control), hipHostMallocDefault)) != hipSuccess)
@@ -686,6 +691,7 @@ void bfs_problem_destroy(bfs_problem *p)
     if (p->d_ctl) (void) hipFree(p->d_ctl);
     if (p->d_hubs) (void) hipFree(p->d_hubs);
     if (p->d_fbm) (void) hipFree(p->d_fbm);
+    if (p->d_marks) (void) hipFree(p->d_marks);
     if (p->h_ctl) (void) hipHostFree(p->h_ctl);
     if (p->ev_start) (void) hipEventDestroy(p->ev_start);
     if (p->ev_stop) (void) hipEventDestroy(p->ev_stop);
@@ -700,7 +706,7 @@ bfs_status bfs_problem_device_info(const bfs_problem *p, int32_t *num_cus, int32
     if (grid_size) *grid_size = persistent_grid(p, kLoopEC, 0);
     if (device_bytes) {
         int64_t b = 4 * ((int64_t) p->nodes + 1) + 4 * p->edges + 4 * (int64_t) p->nodes * (p->mark_pred ? 2 : 1) +
-                    4 * (((int64_t) p->nodes + 31) / 32) + (int64_t) sizeof(Control) +
+                    (4 + 8 + 32) * (((int64_t) p->nodes + 31) / 32) + (int64_t) sizeof(Control) +
                     (int64_t) p->q_cap * 4 * 2 * (p->mark_pred ? 2 : 1);
         *device_bytes = b;
     }

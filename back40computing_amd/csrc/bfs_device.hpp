@@ -344,6 +344,31 @@ struct HubList {
     int chunk;         // edges per chunk
 };
 
+// Index of the last grid-tier entry whose first chunk is <= ch (entries are
+// in chunk order).  A workgroup takes chunks in increasing order, so the
+// search gallops forward from the entry of its previous chunk (`hint`): the
+// next chunk usually lies in the same entry or the next one, one or two
+// loads instead of a binary search's chain of dependent loads.
+__device__ __forceinline__ unsigned hub_entry_of(const HubEntry *list, unsigned entries, unsigned ch, unsigned hint)
+{
+    unsigned lo = 0, hi = entries - 1;
+    if (hint < entries && list[hint].first_chunk <= ch) {
+        lo = hint;
+        unsigned step = 1;
+        while (lo + step < entries && list[lo + step].first_chunk <= ch) {
+            lo += step;
+            step <<= 1;
+        }
+        hi = min(lo + step, entries) - 1;
+    }
+    while (lo < hi) {
+        const unsigned mid = (lo + hi + 1) >> 1;
+        if (list[mid].first_chunk <= ch) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 template <bool PRED, typename Fn>
 __device__ __forceinline__ void gather_rows(int beg, int end, int par, const int *__restrict__ col,
                                             int wave_t, int cta_t, const HubList &hub, Lds<PRED> &lds, Fn &&fn)

@@ -40,6 +40,7 @@ struct SearchArgs {
     HubEntry *hubs[2];     // grid-tier lists, by step & 1
     unsigned hub_cap;      // entries per list
     unsigned *fbm[2];      // frontier bitmaps of the bottom-up levels, by iteration & 1
+    unsigned char *marks;  // [32 * ceil(V / 32)] discovery marks of large top-down levels, clear between steps
     int nodes;
     unsigned long long total_edges;
     int track_edges;       // accumulate frontier edge counts (direction switching)

@@ -234,17 +234,13 @@ __device__ __forceinline__ void hub_phase(const GraphArgs &g, const SearchArgs &
     // guided grabs of consecutive chunks, as for tiles: a level of many hub
     // chunks must not queue on the one chunk counter
     TileGrab tg{0u, 0u, max(1u, min((unsigned) s.grab_max, total / (gridDim.x * 8u)))};
+    unsigned hint = 0;  // thread 0: entry of this workgroup's previous chunk
     while (true) {
         unsigned ch = next_tile(&s.ctl->hub_steal[c.step & 1], lds, tg);
         if (ch >= total) break;
         if (threadIdx.x == 0) {
-            // last entry with first_chunk <= ch (entries are in chunk order)
-            unsigned lo = 0, hi = entries - 1;
-            while (lo < hi) {
-                unsigned mid = (lo + hi + 1) >> 1;
-                if (list[mid].first_chunk <= ch) lo = mid;
-                else hi = mid - 1;
-            }
+            const unsigned lo = hub_entry_of(list, entries, ch, hint);
+            hint = lo;
             HubEntry e = list[lo];
             int b = e.beg + (int) (ch - e.first_chunk) * chunk;
             lds.cta_beg = b;
@@ -659,6 +655,122 @@ __device__ __forceinline__ void frontier_bitmap_pass(const GraphArgs &g, const S
             __hip_atomic_fetch_or(&bm[u >> 5], 1u << (u & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Mark-and-compact form of a large top-down step (expand-contract and the
+// top-down levels of direction-optimizing search).  The claims of a big
+// level are bound by atomic throughput (about 27 G/s on this part, whatever
+// the scope), while plain random byte stores run at about 84 G/s.  Such a
+// level therefore only marks its discoveries: every edge to a vertex whose
+// visited bit is clear (the bits do not change during the pass) stores 1
+// into marks[u] -- duplicates merge, as every writer stores the same byte
+// -- and, for predecessors, the edge's source into pred[u] (any writer
+// leaves a valid parent).  After a grid barrier one streaming pass turns
+// the marks of each 32-vertex word into visited bits, labels and queue
+// entries (in vertex order) and clears them.  The pass and its barrier cost
+// about 10-15 us, so small levels keep the atomic claims.
+// ---------------------------------------------------------------------------
+constexpr unsigned long long kMarkEdges = 1ull << 20;  // fewest frontier edges for the marked form
+
+// fe counts the frontier's edges with multiplicity (best-effort levels may
+// queue a vertex more than once).
+__device__ __forceinline__ bool marked_step(const SearchArgs &s, unsigned long long fe)
+{
+    return s.marks != nullptr && fe >= kMarkEdges && fe >= (unsigned long long) s.nodes / 2;
+}
+
+template <bool PRED, bool EXACT>
+__device__ void body_expand_mark(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds,
+                                 int sub, bool hubs)
+{
+    const int *in = s.q[c.step & 1];
+    auto edge = [&](bool ok, int u, int par) {
+        // Best-effort levels may have lost visited bits of labelled
+        // vertices: the label decides there, or the predecessor store would
+        // overwrite an earlier level's parent.
+        if (ok && !((s.visited[u >> 5] >> (u & 31)) & 1u) && (EXACT || s.depth[u] == -1)) {
+            s.marks[u] = 1;
+            if (PRED) s.pred[u] = par;
+        }
+    };
+    if (sub == 0) {
+        tile_phase<PRED>(g, s, c, n, hub_list(s, c.step, hubs, n), lds, [&](unsigned i, int &beg, int &end, int &par) {
+            int v = ld_queue(in + i);
+            beg = g.row[v];
+            end = g.row[v + 1];
+            par = v;
+        }, edge);
+    } else {
+        hub_phase<PRED>(g, s, c, n, lds, edge);
+    }
+}
+
+// Bits of the four 0/1 bytes of x.
+__device__ __forceinline__ unsigned byte_bits(unsigned x)
+{
+    return (x & 1u) | ((x >> 7) & 2u) | ((x >> 14) & 4u) | ((x >> 21) & 8u);
+}
+
+// Grid-wide; follows the marking pass behind a grid barrier, and a grid
+// barrier must follow it.  Appends to the queue of step c + 1 like the
+// claiming form (one reservation per workgroup and word batch).
+template <bool PRED, bool EXACT>
+__device__ void mark_compact(const GraphArgs &g, const SearchArgs &s, Cursor c, Lds<PRED> &lds)
+{
+    const unsigned words = ((unsigned) g.nodes + 31) >> 5;
+    const int next_depth = c.iteration + 1;
+    int *out = s.q[(c.step + 1) & 1];
+    unsigned fe = 0;
+    bool labeled = false;
+    bool over = false;
+    for (unsigned w0 = blockIdx.x * kBlock; w0 < words; w0 += gridDim.x * kBlock) {
+        const unsigned w = w0 + threadIdx.x;
+        unsigned bits = 0;
+        if (w < words) {
+            uint4 *m = reinterpret_cast<uint4 *>(s.marks + 32ull * w);
+            const uint4 lo = m[0], hi = m[1];
+            bits = byte_bits(lo.x) | byte_bits(lo.y) << 4 | byte_bits(lo.z) << 8 | byte_bits(lo.w) << 12 |
+                   byte_bits(hi.x) << 16 | byte_bits(hi.y) << 20 | byte_bits(hi.z) << 24 | byte_bits(hi.w) << 28;
+            if (bits) {
+                m[0] = make_uint4(0u, 0u, 0u, 0u);
+                m[1] = make_uint4(0u, 0u, 0u, 0u);
+                const unsigned vis = s.visited[w];
+                bits &= ~vis;
+                if (!EXACT) {
+                    // best-effort levels may have lost bits of labelled vertices
+                    for (unsigned b = bits; b; b &= b - 1) {
+                        const int v = (int) ((w << 5) + (unsigned) __builtin_ctz(b));
+                        if (s.depth[v] != -1) bits &= ~(1u << (v & 31));
+                    }
+                }
+                if (bits) s.visited[w] = vis | bits;
+            }
+        }
+        int total = 0;
+        const int excl = block_exclusive_sum(__popc(bits), lds.ep_scan, &total);
+        if (total == 0) continue;  // block-uniform
+        if (threadIdx.x == 0) lds.ep_base = add_agent(&s.ctl->qlen[(c.step + 1) % 3], (unsigned) total);
+        __syncthreads();
+        unsigned pos = lds.ep_base + (unsigned) excl;
+        for (unsigned b = bits; b; b &= b - 1) {
+            const int v = (int) ((w << 5) + (unsigned) __builtin_ctz(b));
+            s.depth[v] = next_depth;
+            if (s.track_edges) fe += (unsigned) (g.row[v + 1] - g.row[v]);
+            if (pos < s.cap) out[pos] = v;
+            else over = true;
+            pos++;
+        }
+        labeled = labeled || bits != 0;
+        __syncthreads();  // ep_base is reused by the next batch
+    }
+    if (over) {
+        st_agent(&s.ctl->overflow, 1u);
+        st_agent(&s.ctl->ovf[(c.step + 1) % 3], 1u);
+    }
+    // frontier-edge, visited-edge and depth counters: one update per workgroup
+    WaveQueue<PRED> wq = make_queue<PRED>(s, c.step, lds);
+    finish_step<PRED>(wq, lds, s.ctl, c.step, labeled, next_depth, fe, s.track_edges != 0);
 }
 
 // Direction choice of a direction-optimizing search at the top of a level
@@ -1236,13 +1348,25 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
         }
 
         unsigned long long t0 = s.instrument ? wall_clock64() : 0;
+        // large top-down levels mark instead of claiming (grid-uniform: fe
+        // is this step's frontier-edge count)
+        const bool mark =
+            (LOOP == kLoopEC || LOOP == kLoopDO) && c.phase == kPhaseExpandContract && marked_step(s, fe);
         if (blockIdx.x == 0 && threadIdx.x == 0) prepare_step(s.ctl, c, n);
-        loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 0, true);
+        if (mark) body_expand_mark<PRED, EXACT>(g, s, c, n, lds, 0, true);
+        else loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 0, true);
         if (s.instrument) busy += wall_clock64() - t0;
         grid_barrier(s.ctl);
         if (gathers(c.phase) && hubs_pending(s, c.step)) {
             t0 = s.instrument ? wall_clock64() : 0;
-            loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 1, true);
+            if (mark) body_expand_mark<PRED, EXACT>(g, s, c, n, lds, 1, true);
+            else loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 1, true);
+            if (s.instrument) busy += wall_clock64() - t0;
+            grid_barrier(s.ctl);
+        }
+        if (mark) {
+            t0 = s.instrument ? wall_clock64() : 0;
+            mark_compact<PRED, EXACT>(g, s, c, lds);
             if (s.instrument) busy += wall_clock64() - t0;
             grid_barrier(s.ctl);
         }

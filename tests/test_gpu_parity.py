@@ -184,6 +184,31 @@ def test_overflow_is_reported_and_recoverable(gpu):
     p.destroy()
 
 
+@pytest.mark.parametrize("cull", [b40c.CULL_EXACT, b40c.CULL_BEST_EFFORT], ids=["exact", "best_effort"])
+@pytest.mark.parametrize("mark_pred", [False, True], ids=["depth", "pred"])
+def test_marked_top_down_levels(gpu, cull, mark_pred):
+    # Levels of at least max(2^20, V/2) frontier edges mark discoveries with
+    # byte stores and compact them behind a barrier instead of claiming;
+    # R-MAT scale 17 (4 M edges) has such levels.  Direction-optimizing
+    # search with a huge alpha stays top-down on every level.
+    g = CsrGraph.rmat(17, 16, seed=0x5A)
+    p = CsrProblem(mark_predecessors=mark_pred).from_host_problem(g)
+    p.set_config(cull_mode=cull)
+    for src in g.sample_sources(2, seed=9):
+        assert_parity(g, p, b40c.EXPAND_CONTRACT, True, src)
+        p.set_config(do_alpha=1 << 20)
+        assert_parity(g, p, b40c.DIRECTION_OPTIMIZING, True, src)
+        p.set_config(do_alpha=8)
+        assert_parity(g, p, b40c.DIRECTION_OPTIMIZING, True, src)
+    # a marked level that overflows the queue reports it, and the problem recovers
+    p.reset(b40c.VERTEX_FRONTIERS, 0.02)
+    with pytest.raises(b40c.BfsError) as ei:
+        p.enact(b40c.EXPAND_CONTRACT, int(g.sample_sources(1, seed=9)[0]))
+    assert ei.value.status == 3  # BFS_ERR_OVERFLOW
+    assert_parity(g, p, b40c.EXPAND_CONTRACT, True, g.sample_sources(1, seed=9)[0])
+    p.destroy()
+
+
 def test_invalid_source_and_frontier_type(gpu):
     g = CsrGraph.grid2d(4)
     p = CsrProblem().from_host_problem(g)
