@@ -112,10 +112,17 @@ __device__ __forceinline__ WaveQueue<PRED> make_queue(const SearchArgs &s, int s
 // instead, in kBlock-edge chunks.  Grid-uniform (n is read after a barrier).
 __device__ __forceinline__ bool narrow_step(unsigned n) { return (n + kBlock - 1) / kBlock < gridDim.x; }
 
-__device__ __forceinline__ int hub_chunk(unsigned n) { return narrow_step(n) ? kBlock : kHubChunk; }
+// Grid-tier chunk of a step over n items.  Marked steps (see
+// body_expand_mark) keep kHubChunk chunks even when narrow: their chunks
+// are gathered four workgroup-widths at a time.
+__device__ __forceinline__ int hub_chunk(unsigned n, bool marked = false)
+{
+    return narrow_step(n) && !marked ? kBlock : kHubChunk;
+}
 
 // Grid-tier list of a gathering step over n input items.
-__device__ __forceinline__ HubList hub_list(const SearchArgs &s, int step, bool enabled, unsigned n)
+__device__ __forceinline__ HubList hub_list(const SearchArgs &s, int step, bool enabled, unsigned n,
+                                            bool marked = false)
 {
     HubList h;
     h.entries = s.hubs[step & 1];
@@ -126,7 +133,7 @@ __device__ __forceinline__ HubList hub_list(const SearchArgs &s, int step, bool 
     // every row long enough for the wave tier goes to the grid instead
     if (enabled && narrow_step(n)) h.grid_t = min(h.grid_t, max(s.wave_t, kWave));
     h.tag = 0;
-    h.chunk = hub_chunk(n);
+    h.chunk = hub_chunk(n, marked);
     return h;
 }
 
@@ -218,9 +225,9 @@ __device__ __forceinline__ void tile_phase(const GraphArgs &g, const SearchArgs 
 // Hub phase: every workgroup takes hub_chunk(n)-edge chunks of the rows that
 // the tile phase published, until all chunks are done.  Must follow a grid
 // barrier (or kernel boundary) after the tile phase.
-template <bool PRED, typename Edge>
-__device__ __forceinline__ void hub_phase(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds,
-                                          Edge &&edge)
+template <bool PRED, typename Chunk>
+__device__ __forceinline__ void hub_phase_chunks(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n,
+                                                 Lds<PRED> &lds, bool marked, Chunk &&run_chunk)
 {
     unsigned long long a = __hip_atomic_load(&s.ctl->hub_alloc[c.step & 1], __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT);
@@ -229,7 +236,7 @@ __device__ __forceinline__ void hub_phase(const GraphArgs &g, const SearchArgs &
     if (entries == 0) return;
     const HubEntry *list = s.hubs[c.step & 1];
     HubEntry lastent = list[entries - 1];
-    const int chunk = hub_chunk(n);
+    const int chunk = hub_chunk(n, marked);
     unsigned total = lastent.first_chunk + (unsigned) ((lastent.end - lastent.beg + chunk - 1) / chunk);
     // guided grabs of consecutive chunks, as for tiles: a level of many hub
     // chunks must not queue on the one chunk counter
@@ -248,14 +255,23 @@ __device__ __forceinline__ void hub_phase(const GraphArgs &g, const SearchArgs &
             lds.cta_par = e.par;
         }
         __syncthreads();
-        int b = lds.cta_beg, e = lds.cta_end, pv = lds.cta_par;
+        run_chunk(lds.cta_beg, lds.cta_end, lds.cta_par);
+    }
+}
+
+
+template <bool PRED, typename Edge>
+__device__ __forceinline__ void hub_phase(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds,
+                                          Edge &&edge)
+{
+    hub_phase_chunks<PRED>(g, s, c, n, lds, false, [&](int b, int e, int pv) {
         for (int k = b; k < e; k += kBlock) {
             int idx = k + threadIdx.x;
             bool ok = idx < e;
             int u = ok ? __builtin_nontemporal_load(g.col + idx) : -1;
             edge(ok, u, pv);
         }
-    }
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -695,15 +711,49 @@ __device__ void body_expand_mark(const GraphArgs &g, const SearchArgs &s, Cursor
         }
     };
     if (sub == 0) {
-        tile_phase<PRED>(g, s, c, n, hub_list(s, c.step, hubs, n), lds, [&](unsigned i, int &beg, int &end, int &par) {
+        tile_phase<PRED>(g, s, c, n, hub_list(s, c.step, hubs, n, true), lds,
+                         [&](unsigned i, int &beg, int &end, int &par) {
             int v = ld_queue(in + i);
             beg = g.row[v];
             end = g.row[v + 1];
             par = v;
         }, edge);
-    } else {
-        hub_phase<PRED>(g, s, c, n, lds, edge);
+        return;
     }
+    // Grid-tier chunks, four workgroup-widths of edges per round with every
+    // load of a round in flight together (the per-edge chain -- column,
+    // visited word, mark -- is the cost of these levels, not bytes).
+    constexpr int U = 4;
+    hub_phase_chunks<PRED>(g, s, c, n, lds, true, [&](int b, int e, int pv) {
+        for (int k = b; k < e; k += U * kBlock) {
+            int u[U];
+            unsigned w[U];
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                const int idx = k + j * kBlock + (int) threadIdx.x;
+                u[j] = idx < e ? __builtin_nontemporal_load(g.col + idx) : -1;
+            }
+#pragma unroll
+            for (int j = 0; j < U; j++) w[j] = u[j] >= 0 ? s.visited[u[j] >> 5] : ~0u;
+            bool todo[U];
+#pragma unroll
+            for (int j = 0; j < U; j++) todo[j] = u[j] >= 0 && !((w[j] >> (u[j] & 31)) & 1u);
+            if (!EXACT) {
+                int dep[U];
+#pragma unroll
+                for (int j = 0; j < U; j++) dep[j] = todo[j] ? s.depth[u[j]] : -1;
+#pragma unroll
+                for (int j = 0; j < U; j++) todo[j] = todo[j] && dep[j] == -1;
+            }
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                if (todo[j]) {
+                    s.marks[u[j]] = 1;
+                    if (PRED) s.pred[u[j]] = pv;
+                }
+            }
+        }
+    });
 }
 
 // Bits of the four 0/1 bytes of x.
@@ -714,13 +764,19 @@ __device__ __forceinline__ unsigned byte_bits(unsigned x)
 
 // Grid-wide; follows the marking pass behind a grid barrier, and a grid
 // barrier must follow it.  Appends to the queue of step c + 1 like the
-// claiming form (one reservation per workgroup and word batch).
+// claiming form (one reservation per workgroup and word batch), in vertex
+// order.  With fbm_out set, also writes every word of that frontier bitmap
+// (the next step's frontier, exactly: a direction-optimizing search that
+// turns bottom-up next needs no bitmap pass).
 template <bool PRED, bool EXACT>
-__device__ void mark_compact(const GraphArgs &g, const SearchArgs &s, Cursor c, Lds<PRED> &lds)
+__device__ void mark_compact(const GraphArgs &g, const SearchArgs &s, Cursor c, Lds<PRED> &lds, unsigned *fbm_out)
 {
     const unsigned words = ((unsigned) g.nodes + 31) >> 5;
     const int next_depth = c.iteration + 1;
     int *out = s.q[(c.step + 1) & 1];
+    // the new vertices of a word batch, staged in the (idle) wave-queue LDS
+    int *stage = &lds.wq[0][0];
+    constexpr int kStage = kWaves * kWaveQueueCap;
     unsigned fe = 0;
     bool labeled = false;
     bool over = false;
@@ -746,23 +802,53 @@ __device__ void mark_compact(const GraphArgs &g, const SearchArgs &s, Cursor c, 
                 }
                 if (bits) s.visited[w] = vis | bits;
             }
+            if (fbm_out) fbm_out[w] = bits;
         }
         int total = 0;
         const int excl = block_exclusive_sum(__popc(bits), lds.ep_scan, &total);
         if (total == 0) continue;  // block-uniform
         if (threadIdx.x == 0) lds.ep_base = add_agent(&s.ctl->qlen[(c.step + 1) % 3], (unsigned) total);
-        __syncthreads();
-        unsigned pos = lds.ep_base + (unsigned) excl;
-        for (unsigned b = bits; b; b &= b - 1) {
-            const int v = (int) ((w << 5) + (unsigned) __builtin_ctz(b));
-            s.depth[v] = next_depth;
-            if (s.track_edges) fe += (unsigned) (g.row[v + 1] - g.row[v]);
-            if (pos < s.cap) out[pos] = v;
-            else over = true;
-            pos++;
-        }
         labeled = labeled || bits != 0;
-        __syncthreads();  // ep_base is reused by the next batch
+        if (total <= kStage) {
+            // labels, queue entries and degrees spread over the workgroup,
+            // four vertices per thread per round with their row loads in
+            // flight together (a thread's own word may hold 32)
+            int k = excl;
+            for (unsigned b = bits; b; b &= b - 1) stage[k++] = (int) ((w << 5) + (unsigned) __builtin_ctz(b));
+            __syncthreads();
+            const unsigned base = lds.ep_base;
+            for (int i0 = 4 * (int) threadIdx.x; i0 < total; i0 += 4 * kBlock) {
+                int v[4], r0[4], r1[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) v[j] = i0 + j < total ? stage[i0 + j] : -1;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    r0[j] = v[j] >= 0 && s.track_edges ? g.row[v[j]] : 0;
+                    r1[j] = v[j] >= 0 && s.track_edges ? g.row[v[j] + 1] : 0;
+                }
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    if (v[j] < 0) continue;
+                    s.depth[v[j]] = next_depth;
+                    const unsigned pos = base + (unsigned) (i0 + j);
+                    if (pos < s.cap) out[pos] = v[j];
+                    else over = true;
+                    fe += (unsigned) (r1[j] - r0[j]);
+                }
+            }
+        } else {
+            __syncthreads();
+            unsigned pos = lds.ep_base + (unsigned) excl;
+            for (unsigned b = bits; b; b &= b - 1) {
+                const int v = (int) ((w << 5) + (unsigned) __builtin_ctz(b));
+                s.depth[v] = next_depth;
+                if (s.track_edges) fe += (unsigned) (g.row[v + 1] - g.row[v]);
+                if (pos < s.cap) out[pos] = v;
+                else over = true;
+                pos++;
+            }
+        }
+        __syncthreads();  // ep_base and the stage are reused by the next batch
     }
     if (over) {
         st_agent(&s.ctl->overflow, 1u);
@@ -1241,6 +1327,11 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
     // label rewrite before any claim.
     bool start = s.init_src >= 0;
     if (start) search_prologue(g, s, LOOP == kLoopDO);
+    // Frontier bitmaps of a direction-optimizing loop (grid-uniform state):
+    // which are not known to be clear, and the step whose frontier one of
+    // them holds exactly (written by a marked step's compaction).
+    unsigned fbm_dirty = start ? 0u : 3u;
+    int fbm_exact_step = -1;
 
     // Every decision at the top of a step reads only values final before
     // the step began and unchanged during it (the input length, the
@@ -1270,15 +1361,19 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
                 start = false;
             }
             if (dir == kPhaseBottomUp && c.phase != kPhaseBottomUp) {
-                if (s.init_src < 0) {  // bitmaps not known to be clear
-                    frontier_bitmap_pass(g, s, c, n, 0);
+                if (fbm_exact_step != c.step) {
+                    if (fbm_dirty & (1u << (c.iteration & 1))) {
+                        frontier_bitmap_pass(g, s, c, n, 0);
+                        grid_barrier(s.ctl);
+                    }
+                    frontier_bitmap_pass(g, s, c, n, 1);
                     grid_barrier(s.ctl);
                 }
-                frontier_bitmap_pass(g, s, c, n, 1);
-                grid_barrier(s.ctl);
+                fbm_dirty = 3u;  // bottom-up levels write both
             } else if (dir != kPhaseBottomUp && c.phase == kPhaseBottomUp) {
-                queue_from_bitmap<PRED>(g, s, c, lds);
+                queue_from_bitmap<PRED>(g, s, c, lds);  // clears both bitmaps
                 grid_barrier(s.ctl);
+                fbm_dirty = 0u;
             }
             c.phase = dir;
         }
@@ -1366,7 +1461,13 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
         }
         if (mark) {
             t0 = s.instrument ? wall_clock64() : 0;
-            mark_compact<PRED, EXACT>(g, s, c, lds);
+            unsigned *fbm_out = nullptr;
+            if (LOOP == kLoopDO) {
+                fbm_out = s.fbm[(c.iteration + 1) & 1];
+                fbm_dirty |= 1u << ((c.iteration + 1) & 1);
+                fbm_exact_step = c.step + 1;
+            }
+            mark_compact<PRED, EXACT>(g, s, c, lds, fbm_out);
             if (s.instrument) busy += wall_clock64() - t0;
             grid_barrier(s.ctl);
         }
