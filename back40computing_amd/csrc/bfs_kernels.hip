@@ -280,7 +280,7 @@ __device__ __forceinline__ void hub_phase(const GraphArgs &g, const SearchArgs &
 // ---------------------------------------------------------------------------
 template <bool PRED, bool EXACT>
 __device__ void body_expand_contract(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds,
-                                     int sub, bool hubs)
+                                     int sub, bool hubs, unsigned *next_bits = nullptr)
 {
     WaveQueue<PRED> wq = make_queue<PRED>(s, c.step, lds);
     const int *in = s.q[c.step & 1];
@@ -294,6 +294,9 @@ __device__ void body_expand_contract(const GraphArgs &g, const SearchArgs &s, Cu
             if (PRED) s.pred[u] = par;
             labeled = true;
             if (s.track_edges) fe += (unsigned) (g.row[u + 1] - g.row[u]);
+            // a clear frontier bitmap becomes the next frontier's, exactly
+            if (next_bits)
+                __hip_atomic_fetch_or(next_bits + (u >> 5), 1u << (u & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         wq.push(fresh, u, par);
     };
@@ -634,7 +637,7 @@ __device__ __forceinline__ void run_body(int phase, const GraphArgs &g, const Se
 // its kernel, which keeps their registers out of its budget).
 template <bool PRED, bool EXACT, int LOOP>
 __device__ __forceinline__ void loop_body(int phase, const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n,
-                                          Lds<PRED> &lds, int sub, bool hubs)
+                                          Lds<PRED> &lds, int sub, bool hubs, unsigned *next_bits = nullptr)
 {
     if constexpr (LOOP == kLoopEC) {
         body_expand_contract<PRED, EXACT>(g, s, c, n, lds, sub, hubs);
@@ -642,7 +645,7 @@ __device__ __forceinline__ void loop_body(int phase, const GraphArgs &g, const S
         body_contract_expand<PRED, EXACT>(g, s, c, n, lds, sub, hubs);
     } else if constexpr (LOOP == kLoopDO) {
         if (phase == kPhaseBottomUp) body_bottom_up<PRED, EXACT>(g, s, c, lds);
-        else body_expand_contract<PRED, EXACT>(g, s, c, n, lds, sub, hubs);
+        else body_expand_contract<PRED, EXACT>(g, s, c, n, lds, sub, hubs, next_bits);
     } else {
         if (phase == kPhaseContract) body_contract<PRED, EXACT>(g, s, c, n, lds);
         else if (phase == kPhaseExpand) body_expand<PRED, EXACT>(g, s, c, n, lds, sub, hubs);
@@ -1447,15 +1450,25 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
         // is this step's frontier-edge count)
         const bool mark =
             (LOOP == kLoopEC || LOOP == kLoopDO) && c.phase == kPhaseExpandContract && marked_step(s, fe);
+        // A top-down step of a direction-optimizing search marks its claims
+        // in the next frontier's bitmap when that bitmap is clear, so a turn
+        // to bottom-up after it needs no bitmap pass.
+        unsigned *next_bits = nullptr;
+        if (LOOP == kLoopDO && !mark && c.phase == kPhaseExpandContract &&
+            !(fbm_dirty & (1u << ((c.iteration + 1) & 1)))) {
+            next_bits = s.fbm[(c.iteration + 1) & 1];
+            fbm_dirty |= 1u << ((c.iteration + 1) & 1);
+            fbm_exact_step = c.step + 1;
+        }
         if (blockIdx.x == 0 && threadIdx.x == 0) prepare_step(s.ctl, c, n);
         if (mark) body_expand_mark<PRED, EXACT>(g, s, c, n, lds, 0, true);
-        else loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 0, true);
+        else loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 0, true, next_bits);
         if (s.instrument) busy += wall_clock64() - t0;
         grid_barrier(s.ctl);
         if (gathers(c.phase) && hubs_pending(s, c.step)) {
             t0 = s.instrument ? wall_clock64() : 0;
             if (mark) body_expand_mark<PRED, EXACT>(g, s, c, n, lds, 1, true);
-            else loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 1, true);
+            else loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 1, true, next_bits);
             if (s.instrument) busy += wall_clock64() - t0;
             grid_barrier(s.ctl);
         }
