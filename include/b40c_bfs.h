@@ -106,10 +106,12 @@ typedef struct bfs_config {
                                        frontier edges * alpha > unexplored edges */
     int32_t do_beta;                /* direction-optimizing: back to top-down when
                                        frontier vertices * beta < V */
-    int32_t tile_grab_max;          /* most 256-item tiles a workgroup takes per
-                                       tile-counter atomic; the grab is
+    int32_t tile_grab_max;          /* most tiles (one item per thread of a workgroup:
+                                       512 single-source, 256 batched) a workgroup
+                                       takes per tile-counter atomic; the grab is
                                        tiles / (grid * 8) clamped to [1, this] */
-    int32_t bu_tile_blocks;         /* batched: 256-vertex blocks per bottom-up tile */
+    int32_t bu_tile_blocks;         /* batched: most 256-vertex blocks per bottom-up
+                                       tile (levels with few blocks use fewer) */
     int32_t grab_strided;           /* batched: 1 = a grab's tiles are gridDim apart
                                        (narrow in-flight window), 2 = consecutive */
     int32_t claim_filter;           /* exact claims: the visited-word test before the
