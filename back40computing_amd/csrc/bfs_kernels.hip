@@ -316,20 +316,36 @@ __device__ void body_expand_contract(const GraphArgs &g, const SearchArgs &s, Cu
 // Bottom-up step of a direction-optimizing search (Beamer, Asanovic,
 // Patterson, "Direction-Optimizing Breadth-First Search", SC'12): every
 // unvisited vertex scans its row for a neighbour in the current frontier
-// bitmap fbm[d & 1] and stops at the first one (in row order).  Vertices go
-// in 64-vertex groups, one lane per vertex, so a group is two whole bitmap
-// words: the visited and next-frontier words are written with plain stores
-// (the next-frontier word is overwritten in full, which also clears it).
+// bitmap fbm[d & 1] and stops at the first one (in row order).
 //
 // The level is bound by each wave's chain of dependent loads (row bounds,
-// neighbours, frontier words), not by bytes, so a wave works on kBuGroups
-// groups at once: every lane scans its kBuGroups rows four neighbours per
-// round, with all rows' loads of a round in flight together.  Rows of at
-// least kBuLong neighbours get one such round (a high-degree vertex usually
-// has a frontier neighbour among its first few); the rest of a long row is
-// scanned by the whole wave, 64 neighbours per round, with a ballot exit.
+// neighbours, frontier words), not by bytes.  Each wave owns one contiguous
+// run of 64-vertex groups (static partition: no tile counter, no workgroup
+// barrier; scrambled ids balance the runs) and walks it in two stages:
+//   1. the visited words and row bounds of kBuGroups groups at a time
+//      (prefetched one step ahead) give the vertices still to search with a
+//      non-empty row; they are queued in the wave's LDS -- on later levels
+//      only a few lanes of a group have work;
+//   2. every 64 queued vertices are scanned with all lanes busy: four
+//      neighbours per lane per round, all loads of a round in flight; rows
+//      of kBuLong+ neighbours get one such round and then a wave-wide scan
+//      of the rest (a high-degree vertex usually finds a frontier neighbour
+//      among its first four).
+// Parents found are collected as bits in LDS per window of kBuWindow groups
+// and written once per word: visited |= found, next frontier = found (the
+// next-frontier word is overwritten in full, which also clears it).
 constexpr int kBuLong = 64;
 constexpr int kBuGroups = 2;
+constexpr int kBuQueue = 128;   // queued vertices per wave (two scans' worth)
+constexpr int kBuWindow = 32;   // groups per window of found bits (2 words each)
+static_assert(3 * kBuQueue + 2 * kBuWindow <= kWaveQueueCap, "bottom-up LDS use fits the wave's staging area");
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <bool PRED, bool EXACT>
 __device__ void body_bottom_up(const GraphArgs &g, const SearchArgs &s, Cursor c, Lds<PRED> &lds)
@@ -338,133 +354,173 @@ __device__ void body_bottom_up(const GraphArgs &g, const SearchArgs &s, Cursor c
     const unsigned *cur = s.fbm[c.iteration & 1];
     unsigned *nxt = s.fbm[(c.iteration + 1) & 1];
     const int lane = lane_id();
+    const int wave = threadIdx.x / kWave;
     const int next_depth = c.iteration + 1;
+    const unsigned words = ((unsigned) g.nodes + 31) >> 5;
     bool labeled = false;
     unsigned fe = 0;
     int found = 0;
-    // Static partition: every wave owns one contiguous run of groups and
-    // walks it with no tile counter and no workgroup barrier, so the waves'
-    // load chains overlap freely.  Vertex ids are scrambled, so the runs
-    // balance without stealing.
+    int *qv = lds.wq[wave];
+    int *qb = qv + kBuQueue;
+    int *qe = qb + kBuQueue;
+    unsigned *fbits = reinterpret_cast<unsigned *>(qe + kBuQueue);
+
     const unsigned groups = ((unsigned) g.nodes + kWave - 1) / kWave;
     const unsigned nwaves = gridDim.x * kWaves;
-    const unsigned gw = blockIdx.x * kWaves + threadIdx.x / kWave;
+    const unsigned gw = blockIdx.x * kWaves + (unsigned) wave;
     const unsigned per = (groups + nwaves - 1) / nwaves;
+    const unsigned g_begin = gw * per;
     const unsigned g_end = min(groups, (gw + 1) * per);
-    // Software pipeline: the visited words and row bounds of the next
-    // kBuGroups groups are in flight while the current ones scan (row bounds
-    // are loaded for visited vertices too: they are contiguous and cheap).
+
+    // Scans the first cnt (1..64, wave-uniform) queued vertices of the
+    // window whose first visited word is word0.
+    auto scan = [&](int cnt, unsigned word0) {
+        int v = -1, beg = 0, end = 0;
+        if (lane < cnt) {
+            v = qv[lane];
+            beg = qb[lane];
+            end = qe[lane];
+        }
+        int pos = beg;
+        const int lim = end - beg < kBuLong ? end : min(end, beg + 4);
+        int parent = -1;
+        while (parent < 0 && pos < lim) {
+            int u[4];
+            unsigned fw[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) u[j] = pos + j < lim ? g.rcol[pos + j] : -1;
+#pragma unroll
+            for (int j = 0; j < 4; j++) fw[j] = u[j] >= 0 ? cur[u[j] >> 5] : 0u;
+#pragma unroll
+            for (int j = 3; j >= 0; j--)  // the first hit in row order wins
+                if (u[j] >= 0 && ((fw[j] >> (u[j] & 31)) & 1u)) parent = u[j];
+            pos += 4;
+        }
+        uint64_t lm = ballot(parent < 0 && lim < end);
+        while (lm) {
+            const int l = uniform(__builtin_ctzll(lm));
+            lm &= lm - 1;
+            const int b = readlane(lim, l), e = readlane(end, l);
+            int hit = -1;
+            for (int q = b; q < e; q += kWave) {
+                const int idx = q + lane;
+                const int uu = idx < e ? g.rcol[idx] : -1;
+                const uint64_t hm = ballot(uu >= 0 && ((cur[uu >> 5] >> (uu & 31)) & 1u));
+                if (hm) {
+                    hit = readlane(uu, uniform(__builtin_ctzll(hm)));
+                    break;
+                }
+            }
+            if (lane == l) parent = hit;
+        }
+        if (parent >= 0) {
+            s.depth[v] = next_depth;
+            if (PRED) s.pred[v] = parent;
+            fe += (unsigned) (end - beg);
+            labeled = true;
+            found++;
+            __hip_atomic_fetch_or(&fbits[((unsigned) v >> 5) - word0], 1u << (v & 31), __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_WAVEFRONT);
+        }
+    };
+    // Moves queue entries [from, from + n) to the front.
+    auto shift = [&](int from, int n) {
+        int v = 0, b = 0, e = 0;
+        if (lane < n) {
+            v = qv[from + lane];
+            b = qb[from + lane];
+            e = qe[from + lane];
+        }
+        wave_lds_sync();
+        if (lane < n) {
+            qv[lane] = v;
+            qb[lane] = b;
+            qe[lane] = e;
+        }
+        wave_lds_sync();
+    };
+
     unsigned vis_n[kBuGroups];
     int beg_n[kBuGroups], end_n[kBuGroups];
-    auto load_groups = [&](unsigned g0) {
+    auto load_groups = [&](unsigned g0, unsigned g_stop) {
 #pragma unroll
         for (int k = 0; k < kBuGroups; k++) {
             const int vv = (int) ((g0 + (unsigned) k) * kWave + (unsigned) lane);
-            const bool ok = g0 + (unsigned) k < g_end && vv < g.nodes;
+            const bool ok = g0 + (unsigned) k < g_stop && vv < g.nodes;
             vis_n[k] = ok ? s.visited[(unsigned) vv >> 5] : ~0u;
             beg_n[k] = ok ? g.rrow[vv] : 0;
             end_n[k] = ok ? g.rrow[vv + 1] : 0;
         }
     };
-    unsigned gi = gw * per;
-    if (gi < g_end) load_groups(gi);
-    for (; gi < g_end; gi += kBuGroups) {
-        unsigned vis[kBuGroups];
-        int beg[kBuGroups], end[kBuGroups], pos[kBuGroups], lim[kBuGroups], parent[kBuGroups];
-        bool todo[kBuGroups];
-#pragma unroll
-        for (int k = 0; k < kBuGroups; k++) {
-            vis[k] = vis_n[k];
-            beg[k] = beg_n[k];
-            end[k] = end_n[k];
-        }
-        if (gi + kBuGroups < g_end) load_groups(gi + kBuGroups);
-#pragma unroll
-        for (int k = 0; k < kBuGroups; k++) {
-            const int v = (int) ((gi + (unsigned) k) * kWave + (unsigned) lane);
-            todo[k] = gi + (unsigned) k < g_end && v < g.nodes && ((vis[k] >> (v & 31)) & 1u) == 0;
-        }
-        if (!EXACT) {
-            // Best-effort top-down levels may have lost visited bits to
-            // racing read-modify-writes; the label stays the authority.
-            int dep[kBuGroups];
-#pragma unroll
-            for (int k = 0; k < kBuGroups; k++)
-                dep[k] = todo[k] ? s.depth[(gi + (unsigned) k) * kWave + (unsigned) lane] : -1;
-#pragma unroll
-            for (int k = 0; k < kBuGroups; k++) todo[k] = todo[k] && dep[k] == -1;
-        }
-#pragma unroll
-        for (int k = 0; k < kBuGroups; k++) {
-            if (!todo[k]) end[k] = beg[k];
-            pos[k] = beg[k];
-            lim[k] = end[k] - beg[k] < kBuLong ? end[k] : min(end[k], beg[k] + 4);
-            parent[k] = -1;
-        }
-        // per-lane rounds of four neighbours of every row still searching
-        while (true) {
-            bool more = false;
-#pragma unroll
-            for (int k = 0; k < kBuGroups; k++) more |= parent[k] < 0 && pos[k] < lim[k];
-            if (!more) break;
-            int u[kBuGroups][4];
-#pragma unroll
-            for (int k = 0; k < kBuGroups; k++)
-#pragma unroll
-                for (int j = 0; j < 4; j++)
-                    u[k][j] = parent[k] < 0 && pos[k] + j < lim[k] ? g.rcol[pos[k] + j] : -1;
-            unsigned fw[kBuGroups][4];
-#pragma unroll
-            for (int k = 0; k < kBuGroups; k++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) fw[k][j] = u[k][j] >= 0 ? cur[u[k][j] >> 5] : 0u;
+    if (g_begin < g_end) load_groups(g_begin, g_end);
+    for (unsigned w_begin = g_begin; w_begin < g_end; w_begin += kBuWindow) {
+        const unsigned w_end = min(g_end, w_begin + kBuWindow);
+        const unsigned word0 = 2 * w_begin;
+        fbits[lane] = 0u;  // 64 lanes, 2 * kBuWindow words
+        wave_lds_sync();
+        int qn = 0;
+        for (unsigned gi = w_begin; gi < w_end; gi += kBuGroups) {
+            unsigned vis[kBuGroups];
+            int beg[kBuGroups], end[kBuGroups];
 #pragma unroll
             for (int k = 0; k < kBuGroups; k++) {
-#pragma unroll
-                for (int j = 3; j >= 0; j--)  // the first hit in row order wins
-                    if (u[k][j] >= 0 && ((fw[k][j] >> (u[k][j] & 31)) & 1u)) parent[k] = u[k][j];
-                pos[k] += 4;
+                vis[k] = vis_n[k];
+                beg[k] = beg_n[k];
+                end[k] = end_n[k];
             }
-        }
-        // the rest of the long rows, one row at a time by the whole wave
+            // the next groups of this run (the window boundary does not stop the pipeline)
+            if (gi + kBuGroups < g_end) load_groups(gi + kBuGroups, g_end);
+            bool todo[kBuGroups];
 #pragma unroll
-        for (int k = 0; k < kBuGroups; k++) {
-            uint64_t lm = ballot(parent[k] < 0 && lim[k] < end[k]);
-            while (lm) {
-                const int l = uniform(__builtin_ctzll(lm));
-                lm &= lm - 1;
-                const int b = readlane(lim[k], l), e = readlane(end[k], l);
-                int hit = -1;
-                for (int q = b; q < e; q += kWave) {
-                    const int idx = q + lane;
-                    const int uu = idx < e ? g.rcol[idx] : -1;
-                    const uint64_t hm = ballot(uu >= 0 && ((cur[uu >> 5] >> (uu & 31)) & 1u));
-                    if (hm) {
-                        hit = readlane(uu, uniform(__builtin_ctzll(hm)));
-                        break;
-                    }
+            for (int k = 0; k < kBuGroups; k++) {
+                const int v = (int) ((gi + (unsigned) k) * kWave + (unsigned) lane);
+                todo[k] = gi + (unsigned) k < w_end && v < g.nodes && ((vis[k] >> (v & 31)) & 1u) == 0 &&
+                          end[k] > beg[k];
+            }
+            if (!EXACT) {
+                // Best-effort top-down levels may have lost visited bits to
+                // racing read-modify-writes; the label stays the authority.
+                int dep[kBuGroups];
+#pragma unroll
+                for (int k = 0; k < kBuGroups; k++)
+                    dep[k] = todo[k] ? s.depth[(gi + (unsigned) k) * kWave + (unsigned) lane] : -1;
+#pragma unroll
+                for (int k = 0; k < kBuGroups; k++) todo[k] = todo[k] && dep[k] == -1;
+            }
+#pragma unroll
+            for (int k = 0; k < kBuGroups; k++) {
+                const uint64_t m = ballot(todo[k]);
+                if (todo[k]) {
+                    const int p = qn + lanes_below(m);
+                    qv[p] = (int) ((gi + (unsigned) k) * kWave + (unsigned) lane);
+                    qb[p] = beg[k];
+                    qe[p] = end[k];
                 }
-                if (lane == l) parent[k] = hit;
+                qn += __popcll(m);
+                if (qn >= kWave) {
+                    wave_lds_sync();
+                    scan(kWave, word0);
+                    shift(kWave, qn - kWave);
+                    qn -= kWave;
+                }
             }
         }
-#pragma unroll
-        for (int k = 0; k < kBuGroups; k++) {
-            const unsigned v = (gi + (unsigned) k) * kWave + (unsigned) lane;
-            const bool got = parent[k] >= 0;
-            if (got) {
-                s.depth[v] = next_depth;
-                if (PRED) s.pred[v] = parent[k];
-                fe += (unsigned) (end[k] - beg[k]);
-                labeled = true;
-            }
-            const uint64_t gm = ballot(got);
-            if ((lane & 31) == 0 && gi + (unsigned) k < g_end && (int) v < g.nodes) {
-                const unsigned bits = (unsigned) (lane == 0 ? gm : gm >> 32);
-                if (bits) s.visited[v >> 5] = vis[k] | bits;
-                nxt[v >> 5] = bits;
-            }
-            found += got ? 1 : 0;
+        wave_lds_sync();
+        while (qn > 0) {
+            const int cnt = min(qn, kWave);
+            scan(cnt, word0);
+            if (qn > cnt) shift(cnt, qn - cnt);
+            qn -= cnt;
         }
+        wave_lds_sync();
+        // the window's visited and next-frontier words (the wave owns them)
+        const unsigned w = word0 + (unsigned) lane;
+        if (lane < 2 * (int) (w_end - w_begin) && w < words) {
+            const unsigned bits = fbits[lane];
+            if (bits) s.visited[w] |= bits;
+            nxt[w] = bits;
+        }
+        wave_lds_sync();
     }
     // The level's output is the bitmap nxt; only its size goes to the queue
     // length (queue_from_bitmap materialises the queue if the search turns
