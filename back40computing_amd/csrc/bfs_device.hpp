@@ -543,5 +543,21 @@ __device__ __forceinline__ void grid_barrier(Control *ctl)
     __syncthreads();
 }
 
+// Thread 0 of block 0 only: waits until every other workgroup has arrived at
+// the grid barrier in progress (block 0 itself has not), i.e. until every
+// other workgroup is done with whatever it did before that barrier.  Leaf 0
+// then holds all its workgroups but block 0 and the root all leaves but
+// leaf 0; neither count can move on before block 0 arrives, so the wait
+// ends exactly once they are reached.
+__device__ __forceinline__ void wait_other_arrivals(Control *ctl)
+{
+    const unsigned nblocks = gridDim.x;
+    const unsigned leaves = nblocks < (unsigned) kBarrierFanIn ? nblocks : (unsigned) kBarrierFanIn;
+    const unsigned leaf0 = (nblocks + kBarrierFanIn - 1) / kBarrierFanIn;  // workgroups on leaf 0
+    while (__hip_atomic_load(&ctl->bar_root, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != leaves - 1 ||
+           __hip_atomic_load(&ctl->bar_leaf[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != leaf0 - 1)
+        __builtin_amdgcn_s_sleep(2);
+}
+
 }  // inline namespace (workgroup size)
 }  // namespace b40c_hip

@@ -1218,7 +1218,7 @@ __device__ void fast_lds_level(const GraphArgs &g, const SearchArgs &s, Cursor c
 // only; every thread calls it.
 template <bool PRED, bool EXACT>
 __device__ void fast_lds_path(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, int loop,
-                              unsigned long long fv, FastFront &F)
+                              unsigned long long fv, FastFront &F, bool wait_others)
 {
     const int *in = s.q[c.step & 1];
     for (unsigned i = threadIdx.x; i < n; i += kBlock) {
@@ -1256,6 +1256,13 @@ __device__ void fast_lds_path(const GraphArgs &g, const SearchArgs &s, Cursor c,
         n = n2;
         if (!go) break;
         __syncthreads();  // every thread has read F's results before the next level resets them
+    }
+    // The hand-off below rewrites counters the other workgroups may still
+    // be reading at the top of the step the path started from, unless they
+    // have all arrived at the barrier that ends the path.
+    if (wait_others) {
+        if (threadIdx.x == 0) wait_other_arrivals(s.ctl);
+        __syncthreads();
     }
     // The next step (c) reads its input from the global queue: copy the
     // LDS-resident members (those past kFastCap were written there already).
@@ -1440,23 +1447,30 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
         const bool fast = fast_level(s, c, n, fe) && gridDim.x > 1 &&
                           ((LOOP != kLoopEC && LOOP != kLoopDO) || (n <= (unsigned) kFastCap && fe <= kFastEdges));
         if (start && !fast) grid_barrier(s.ctl);
+        const bool first = start;
         start = false;
         if (fast) {
             // Single-workgroup fast path: block 0 runs consecutive small
             // levels with workgroup barriers only (no grid tier: it alone
             // works); the rest of the grid waits at the next grid barrier.
-            // The entry barrier makes every workgroup finish reading this
-            // step's counters first: block 0's second fast step zeroes
+            // Every workgroup must be done reading this step's counters
+            // before block 0 rewrites them (its second fast step zeroes
             // qlen / fedges of index step % 3, which a workgroup still on
-            // its way out of the previous barrier would otherwise read as
-            // an empty frontier and leave the loop, stranding the grid at
-            // the barrier below.
-            grid_barrier(s.ctl);
+            // its way out of the previous barrier would otherwise read as an
+            // empty frontier and leave the loop, stranding the grid at the
+            // barrier below).  The per-level path rewrites counters as it
+            // goes and takes an entry barrier.  The LDS path writes them
+            // only when it ends, so it starts at once and, before writing,
+            // waits for the others' arrival at the closing barrier -- except
+            // after a folded search start, whose label rewrite the entry
+            // barrier orders before the first claim.
+            const bool lds_path = LOOP == kLoopEC || LOOP == kLoopDO;
+            if (!lds_path || first) grid_barrier(s.ctl);
             if constexpr (LOOP == kLoopEC || LOOP == kLoopDO) {
                 // frontier kept in LDS between levels
                 if (blockIdx.x == 0) {
                     unsigned long long t0 = s.instrument ? wall_clock64() : 0;
-                    fast_lds_path<PRED, EXACT>(g, s, c, n, loop, vsum, shared.fast);
+                    fast_lds_path<PRED, EXACT>(g, s, c, n, loop, vsum, shared.fast, lds_path && !first);
                     __syncthreads();
                     init_lds(lds);
                     if (s.instrument) busy += wall_clock64() - t0;

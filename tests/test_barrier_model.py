@@ -62,3 +62,36 @@ def test_fixed_protocols_always_release():
                 grids = [(n, b) for n, b in zip(sizes, (1, 2, 3))]
                 stuck = launches(release, grids, lambda k, i, n: last % n, protocol=protocol)
                 assert stuck == [], (protocol, sizes, last)
+
+
+def others_arrived(root, leaf_counts, nblocks):
+    """The test wait_other_arrivals (bfs_device.hpp) spins on."""
+    leaves = min(nblocks, FAN_IN)
+    leaf0 = (nblocks + FAN_IN - 1) // FAN_IN
+    return root == leaves - 1 and leaf_counts[0] == leaf0 - 1
+
+
+def test_wait_for_other_arrivals_predicate():
+    # Arrivals as grid_barrier performs them: a workgroup increments its
+    # leaf's counter; the leaf's last arriver resets it and increments the
+    # root.  Block 0 has not arrived.  The predicate must hold exactly once
+    # every other workgroup has arrived, in any arrival order.
+    import random
+    rng = random.Random(7)
+    for nblocks in list(range(2, 40)) + [255, 256, 512, 768]:
+        leaves = min(nblocks, FAN_IN)
+        for trial in range(20):
+            order = list(range(1, nblocks))
+            rng.shuffle(order)
+            leaf_counts = [0] * FAN_IN
+            root = 0
+            for i, b in enumerate(order):
+                assert not others_arrived(root, leaf_counts, nblocks), (nblocks, i)
+                leaf = b % FAN_IN
+                size = (nblocks - leaf + FAN_IN - 1) // FAN_IN
+                leaf_counts[leaf] += 1
+                if leaf_counts[leaf] == size:
+                    leaf_counts[leaf] = 0
+                    root += 1
+            assert others_arrived(root, leaf_counts, nblocks), nblocks
+            assert root <= leaves - 1
