@@ -369,6 +369,37 @@ __device__ __forceinline__ unsigned hub_entry_of(const HubEntry *list, unsigned 
     return lo;
 }
 
+// Grid-tier publication of a row of at least hub.grid_t neighbours (called
+// wave-converged; true when this lane's row was published).  The wave
+// reserves list entries and chunk numbers for all its long rows with one
+// atomic; entries keep lane order, so the list stays sorted by first chunk.
+__device__ __forceinline__ bool publish_hub_row(int beg, int end, int par, const HubList &hub)
+{
+    const int lane = lane_id();
+    const int len = end - beg;
+    const bool pub = len >= hub.grid_t;
+    const uint64_t pm = ballot(pub);
+    if (!pm) return false;
+    const int nch = pub ? (len + hub.chunk - 1) / hub.chunk : 0;
+    const int incl = wave_inclusive_sum(nch);
+    const unsigned total = (unsigned) readlane(incl, kWave - 1);
+    unsigned lo = 0, hi = 0;
+    if (lane == 0) {
+        unsigned long long old = __hip_atomic_fetch_add(hub.alloc, ((unsigned long long) __popcll(pm) << 32) | total,
+                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        lo = (unsigned) old;
+        hi = (unsigned) (old >> 32);
+    }
+    lo = (unsigned) uniform((int) lo);
+    hi = (unsigned) uniform((int) hi);
+    const unsigned slot = hi + (unsigned) lanes_below(pm);
+    if (pub && slot < hub.cap) {
+        hub.entries[slot] = HubEntry{beg, end, par, lo + (unsigned) (incl - nch), hub.tag};
+        return true;
+    }
+    return false;
+}
+
 template <bool PRED, typename Fn>
 __device__ __forceinline__ void gather_rows(int beg, int end, int par, const int *__restrict__ col,
                                             int wave_t, int cta_t, const HubList &hub, Lds<PRED> &lds, Fn &&fn)
@@ -379,34 +410,9 @@ __device__ __forceinline__ void gather_rows(int beg, int end, int par, const int
     int len = end - beg;
 
     // Grid tier: publish the row as chunks for every workgroup of the grid
-    // (processed after this tile phase).  The wave reserves list entries and
-    // chunk numbers for all its long rows with one atomic; entries keep
-    // lane order, so the list stays sorted by first chunk.  A full list
-    // falls back to the CTA tier below.
-    {
-        const bool pub = len >= hub.grid_t;
-        const uint64_t pm = ballot(pub);
-        if (pm) {
-            const int nch = pub ? (len + hub.chunk - 1) / hub.chunk : 0;
-            const int incl = wave_inclusive_sum(nch);
-            const unsigned total = (unsigned) readlane(incl, kWave - 1);
-            unsigned lo = 0, hi = 0;
-            if (lane == 0) {
-                unsigned long long old = __hip_atomic_fetch_add(
-                    hub.alloc, ((unsigned long long) __popcll(pm) << 32) | total, __ATOMIC_RELAXED,
-                    __HIP_MEMORY_SCOPE_AGENT);
-                lo = (unsigned) old;
-                hi = (unsigned) (old >> 32);
-            }
-            lo = (unsigned) uniform((int) lo);
-            hi = (unsigned) uniform((int) hi);
-            const unsigned slot = hi + (unsigned) lanes_below(pm);
-            if (pub && slot < hub.cap) {
-                hub.entries[slot] = HubEntry{beg, end, par, lo + (unsigned) (incl - nch), hub.tag};
-                len = 0;
-            }
-        }
-    }
+    // (processed after this tile phase).  A full list falls back to the CTA
+    // tier below.
+    if (publish_hub_row(beg, end, par, hub)) len = 0;
 
     // CTA tier
     while (true) {

@@ -770,13 +770,85 @@ __device__ void body_expand_mark(const GraphArgs &g, const SearchArgs &s, Cursor
         }
     };
     if (sub == 0) {
-        tile_phase<PRED>(g, s, c, n, hub_list(s, c.step, hubs, n, true), lds,
-                         [&](unsigned i, int &beg, int &end, int &par) {
-            int v = ld_queue(in + i);
-            beg = g.row[v];
-            end = g.row[v + 1];
-            par = v;
-        }, edge);
+        // Tiles of kBlock frontier vertices: rows of the grid tier are
+        // published, the rest of the tile's edges are spread evenly over the
+        // workgroup through an LDS degree scan (as in the LDS fast path),
+        // four consecutive edges per thread per round with every load of a
+        // round in flight -- instead of the gather tiers' one chain per
+        // round of 64 edges.
+        const HubList hub = hub_list(s, c.step, hubs, n, true);
+        int *tbeg = lds.wq[0];  // the wave-queue staging is idle in a marked step
+        int *tpar = lds.wq[1];
+        int *toff = lds.wq[2];
+        static_assert(kWaveQueueCap >= kBlock && kWaves >= 3, "tile tables fit the staging area");
+        TileGrab tg = tile_grab(s, n);
+        while (true) {
+            const unsigned base = next_tile(&s.ctl->steal[c.step & 1], lds, tg) * kBlock;
+            if (base >= n) break;
+            const unsigned i = base + threadIdx.x;
+            int v = -1, beg = 0, end = 0;
+            if (i < n) {
+                v = ld_queue(in + i);
+                beg = g.row[v];
+                end = g.row[v + 1];
+            }
+            if (publish_hub_row(beg, end, v, hub)) end = beg;
+            int total = 0;
+            const int off = block_exclusive_sum(end - beg, lds.ep_scan, &total);
+            // (the scan's barriers order the previous tile's reads before these writes)
+            tbeg[threadIdx.x] = beg;
+            tpar[threadIdx.x] = v;
+            toff[threadIdx.x] = off;
+            __syncthreads();
+            for (int b0 = 0; b0 < total; b0 += kBlock * 4) {
+                const int e0 = b0 + (int) threadIdx.x * 4;
+                int r = 0;
+                if (e0 < total) {  // last row with off <= e0 (it has edges)
+                    int lo = 0, hi = kBlock - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (toff[mid] <= e0) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    r = lo;
+                }
+                int at[4], pv[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int e = e0 + j;
+                    at[j] = -1;
+                    pv[j] = -1;
+                    if (e < total) {
+                        while (r + 1 < kBlock && toff[r + 1] <= e) r++;
+                        at[j] = tbeg[r] + (e - toff[r]);
+                        pv[j] = tpar[r];
+                    }
+                }
+                int u[4];
+                unsigned w[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) u[j] = at[j] >= 0 ? __builtin_nontemporal_load(g.col + at[j]) : -1;
+#pragma unroll
+                for (int j = 0; j < 4; j++) w[j] = u[j] >= 0 ? s.visited[u[j] >> 5] : ~0u;
+                bool todo[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) todo[j] = u[j] >= 0 && !((w[j] >> (u[j] & 31)) & 1u);
+                if (!EXACT) {
+                    int dep[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) dep[j] = todo[j] ? s.depth[u[j]] : -1;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) todo[j] = todo[j] && dep[j] == -1;
+                }
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    if (todo[j]) {
+                        s.marks[u[j]] = 1;
+                        if (PRED) s.pred[u[j]] = pv[j];
+                    }
+                }
+            }
+        }
         return;
     }
     // Grid-tier chunks, four workgroup-widths of edges per round with every
