@@ -759,16 +759,11 @@ template <bool PRED, bool EXACT>
 __device__ void body_expand_mark(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds,
                                  int sub, bool hubs)
 {
+    // An edge marks its target when the target's visited bit is clear and,
+    // in best-effort mode, its label is still -1: that mode may have lost
+    // visited bits of labelled vertices, whose predecessors a mark would
+    // overwrite.
     const int *in = s.q[c.step & 1];
-    auto edge = [&](bool ok, int u, int par) {
-        // Best-effort levels may have lost visited bits of labelled
-        // vertices: the label decides there, or the predecessor store would
-        // overwrite an earlier level's parent.
-        if (ok && !((s.visited[u >> 5] >> (u & 31)) & 1u) && (EXACT || s.depth[u] == -1)) {
-            s.marks[u] = 1;
-            if (PRED) s.pred[u] = par;
-        }
-    };
     if (sub == 0) {
         // Tiles of kBlock frontier vertices: rows of the grid tier are
         // published, the rest of the tile's edges are spread evenly over the
