@@ -460,7 +460,8 @@ __device__ void batch_bu_tile(const BatchArgs &a, BatchLds &L, int q, unsigned l
             const bool got = parent[k] >= 0;
             if (got) {
                 sl.depth[v] = d + 1;
-                o.fe += (unsigned) (g.row[v + 1] - g.row[v]);
+                // out-degree: the scanned in-row's length when the graph is symmetric
+                o.fe += (unsigned) (g.rrow == g.row ? end[k] - beg[k] : g.row[v + 1] - g.row[v]);
             }
             const uint64_t gm = ballot(got);
             if ((lane & 31) == 0 && valid[k]) {
