@@ -339,138 +339,177 @@ __device__ __forceinline__ bool batch_hubs_pending(const BatchArgs &a, int level
 // written by a bottom-up previous level, or depth[u] == current depth after
 // a top-down one (a vertex labelled during this level holds depth + 1, or
 // still -1 if its store is not yet visible, and fails either way).
-//
-// As in the single-source bottom-up step, the level is bound by dependent
-// load chains: a wave scans kBatchBuGroups blocks at once, four neighbours
-// of every row per round with all loads of a round in flight (the visited
-// word and the label of a neighbour are read together), long rows get one
-// such round before the wave-wide scan of the rest.
-constexpr int kBatchBuGroups = 1;
-constexpr int kBatchBuLong = 64;
+__device__ __forceinline__ void wave_lds_sync_b()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Queued form of a bottom-up tile (as the single-source step): each wave
+// queues the vertices of its slice of the tile's blocks that still search a
+// parent and have a non-empty row, and scans them 64 at a time with every
+// lane busy; parents found become bits per window of kBatchBuWindow blocks
+// (two words per wave and block) written once per word.  The wave's queue
+// lives in LDS the batch kernels leave idle during a tile: the history-hash
+// area and the scan-tier staging of gather_rows (used only inside a call).
+constexpr int kBatchBuWindow = 32;
 
 __device__ void batch_bu_tile(const BatchArgs &a, BatchLds &L, int q, unsigned local, SlotOut &o)
 {
+    static_assert(kHistory >= kWaves * 256, "per-wave queue of 128 vertices and row starts");
     BatchSlot &sl = a.slots[q];
     const BatchGraph g = a.graphs[sl.graph];
     const int d = L.it[q];
     const int lane = lane_id();
-    WaveQueue<false> &wq = o.wq;
+    const int wave = threadIdx.x / kWave;
     const bool use_bitmap = L.bits[q] != 0;
+    const bool symmetric = g.rrow == g.row;
     const unsigned *cur = sl.fbm[d & 1];
     unsigned *nxt = sl.fbm[(d + 1) & 1];
+    const unsigned words = ((unsigned) g.nodes + 31) >> 5;
     const unsigned nblocks = (unsigned) (g.nodes + kBlock - 1) / kBlock;
     const unsigned first = local * (unsigned) L.bu_tile;
     const unsigned last = min(nblocks, first + (unsigned) L.bu_tile);
-    constexpr int G = kBatchBuGroups;
-    // software pipeline: the next blocks' visited words and row bounds are
-    // in flight while the current ones scan
-    unsigned vis_n[G];
-    int beg_n[G], end_n[G];
-    auto load_blocks = [&](unsigned b) {
-#pragma unroll
-        for (int k = 0; k < G; k++) {
-            const int v = (int) ((b + (unsigned) k) * kBlock + threadIdx.x);
-            const bool ok = b + (unsigned) k < last && v < g.nodes;
-            vis_n[k] = ok ? sl.visited[(unsigned) v >> 5] : ~0u;
-            beg_n[k] = ok ? g.rrow[v] : 0;
-            end_n[k] = ok ? g.rrow[v + 1] : 0;
-        }
+    int *qv = L.base.hist + wave * 256;  // [128] vertices
+    int *qb = qv + 128;                  // [128] row starts
+    int *qe0 = L.base.sc_excl[wave];     // row ends, entries 0..63
+    int *qe1 = L.base.sc_beg[wave];      // row ends, entries 64..127
+    unsigned *fb = reinterpret_cast<unsigned *>(L.base.sc_par[wave]);  // found bits of the window
+
+    auto in_frontier = [&](int u) {
+        if (use_bitmap) return ((cur[u >> 5] >> (u & 31)) & 1u) != 0;
+        return ((sl.visited[u >> 5] >> (u & 31)) & 1u) != 0 && sl.depth[u] == d;
     };
-    if (first < last) load_blocks(first);
-    for (unsigned b0 = first; b0 < last; b0 += G) {
-        unsigned vis[G];
-        int beg[G], end[G], pos[G], lim[G], parent[G];
-        bool todo[G], valid[G];
-#pragma unroll
-        for (int k = 0; k < G; k++) {
-            const int v = (int) ((b0 + (unsigned) k) * kBlock + threadIdx.x);
-            valid[k] = b0 + (unsigned) k < last && v < g.nodes;
-            vis[k] = vis_n[k];
-            beg[k] = beg_n[k];
-            end[k] = end_n[k];
+    // Scans the first cnt (1..64) queued vertices of the window starting at block wb0.
+    auto scan = [&](int cnt, unsigned wb0) {
+        int v = -1, beg = 0, end = 0;
+        if (lane < cnt) {
+            v = qv[lane];
+            beg = qb[lane];
+            end = qe0[lane];
         }
-        if (b0 + G < last) load_blocks(b0 + G);
+        int pos = beg;
+        const int lim = end - beg < 64 ? end : min(end, beg + 4);
+        int parent = -1;
+        while (parent < 0 && pos < lim) {
+            int u[4];
+            unsigned fw[4];
+            int lab[4];
 #pragma unroll
-        for (int k = 0; k < G; k++) {
-            const int v = (int) ((b0 + (unsigned) k) * kBlock + threadIdx.x);
-            todo[k] = valid[k] && ((vis[k] >> (v & 31)) & 1u) == 0;
-            if (!todo[k]) end[k] = beg[k];
-            pos[k] = beg[k];
-            lim[k] = end[k] - beg[k] < kBatchBuLong ? end[k] : min(end[k], beg[k] + 4);
-            parent[k] = -1;
+            for (int j = 0; j < 4; j++) u[j] = pos + j < lim ? g.rcol[pos + j] : -1;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                fw[j] = u[j] >= 0 ? (use_bitmap ? cur[u[j] >> 5] : sl.visited[u[j] >> 5]) : 0u;
+                lab[j] = u[j] >= 0 && !use_bitmap ? sl.depth[u[j]] : d;
+            }
+#pragma unroll
+            for (int j = 3; j >= 0; j--)  // the first hit in row order wins
+                if (u[j] >= 0 && ((fw[j] >> (u[j] & 31)) & 1u) && lab[j] == d) parent = u[j];
+            pos += 4;
         }
+        uint64_t lm = ballot(parent < 0 && lim < end);
+        while (lm) {
+            const int l = uniform(__builtin_ctzll(lm));
+            lm &= lm - 1;
+            const int b = readlane(lim, l), e = readlane(end, l);
+            int hit = -1;
+            for (int r = b; r < e; r += kWave) {
+                const int idx = r + lane;
+                const int uu = idx < e ? g.rcol[idx] : -1;
+                const uint64_t hm = ballot(uu >= 0 && in_frontier(uu));
+                if (hm) {
+                    hit = readlane(uu, uniform(__builtin_ctzll(hm)));
+                    break;
+                }
+            }
+            if (lane == l) parent = hit;
+        }
+        const bool got = parent >= 0;
+        if (got) {
+            sl.depth[v] = d + 1;
+            // out-degree: the scanned in-row's length when the graph is symmetric
+            o.fe += (unsigned) (symmetric ? end - beg : g.row[v + 1] - g.row[v]);
+            const unsigned wl = 2u * (((unsigned) v / kBlock) - wb0) + (((unsigned) v >> 5) & 1u);
+            __hip_atomic_fetch_or(&fb[wl], 1u << (v & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        }
+        o.wq.push(got, v, parent);
+    };
+    // Moves queue entries [64, 64 + n) (n < 64) to the front.
+    auto shift = [&](int n) {
+        int v = 0, b = 0, e = 0;
+        if (lane < n) {
+            v = qv[kWave + lane];
+            b = qb[kWave + lane];
+            e = qe1[lane];
+        }
+        wave_lds_sync_b();
+        if (lane < n) {
+            qv[lane] = v;
+            qb[lane] = b;
+            qe0[lane] = e;
+        }
+        wave_lds_sync_b();
+    };
+
+    // software pipeline: the next block's visited word and row bounds are
+    // in flight while the current one is queued and scanned
+    unsigned vis_n = ~0u;
+    int beg_n = 0, end_n = 0;
+    auto load_block = [&](unsigned b) {
+        const int v = (int) (b * kBlock + threadIdx.x);
+        const bool ok = b < last && v < g.nodes;
+        vis_n = ok ? sl.visited[(unsigned) v >> 5] : ~0u;
+        beg_n = ok ? g.rrow[v] : 0;
+        end_n = ok ? g.rrow[v + 1] : 0;
+    };
+    if (first < last) load_block(first);
+    for (unsigned wb0 = first; wb0 < last; wb0 += kBatchBuWindow) {
+        const unsigned wb_end = min(last, wb0 + kBatchBuWindow);
+        fb[lane] = 0u;  // 64 lanes: two words per block of the window
+        wave_lds_sync_b();
+        // one loop with a single scan site (inlining the scan twice costs
+        // registers, and with them workgroups per CU): queue the next block
+        // while fewer than 64 wait and blocks remain, scan otherwise
+        int qn = 0;
+        unsigned b = wb0;
         while (true) {
-            bool more = false;
-#pragma unroll
-            for (int k = 0; k < G; k++) more |= parent[k] < 0 && pos[k] < lim[k];
-            if (!more) break;
-            int u[G][4];
-#pragma unroll
-            for (int k = 0; k < G; k++)
-#pragma unroll
-                for (int j = 0; j < 4; j++)
-                    u[k][j] = parent[k] < 0 && pos[k] + j < lim[k] ? g.rcol[pos[k] + j] : -1;
-            unsigned fw[G][4];
-            int lab[G][4];
-#pragma unroll
-            for (int k = 0; k < G; k++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int uu = u[k][j];
-                    fw[k][j] = uu >= 0 ? (use_bitmap ? cur[uu >> 5] : sl.visited[uu >> 5]) : 0u;
-                    lab[k][j] = uu >= 0 && !use_bitmap ? sl.depth[uu] : d;
-                }
-#pragma unroll
-            for (int k = 0; k < G; k++) {
-#pragma unroll
-                for (int j = 3; j >= 0; j--)  // the first hit in row order wins
-                    if (u[k][j] >= 0 && ((fw[k][j] >> (u[k][j] & 31)) & 1u) && lab[k][j] == d) parent[k] = u[k][j];
-                pos[k] += 4;
+            if (qn >= kWave || (b >= wb_end && qn > 0)) {
+                const int cnt = min(qn, kWave);
+                wave_lds_sync_b();
+                scan(cnt, wb0);
+                if (qn > cnt) shift(qn - cnt);
+                qn -= cnt;
+                continue;
+            }
+            if (b >= wb_end) break;
+            const unsigned vis = vis_n;
+            const int beg = beg_n, end = end_n;
+            if (b + 1 < last) load_block(b + 1);
+            const int v = (int) (b * kBlock + threadIdx.x);
+            const bool todo = v < g.nodes && ((vis >> (v & 31)) & 1u) == 0 && end > beg;
+            const uint64_t m = ballot(todo);
+            if (todo) {
+                const int p = qn + lanes_below(m);
+                qv[p] = v;
+                qb[p] = beg;
+                if (p < kWave) qe0[p] = end;
+                else qe1[p - kWave] = end;
+            }
+            qn += __popcll(m);
+            b++;
+        }
+        wave_lds_sync_b();
+        // the window's visited and next-frontier words of this wave's slices
+        if (lane < 2 * (int) (wb_end - wb0)) {
+            const unsigned w = (wb0 + (unsigned) lane / 2u) * (kBlock / 32) + 2u * (unsigned) wave + ((unsigned) lane & 1u);
+            if (w < words) {
+                const unsigned bits = fb[lane];
+                if (bits) sl.visited[w] |= bits;
+                nxt[w] = bits;  // whole word: also clears it
             }
         }
-#pragma unroll
-        for (int k = 0; k < G; k++) {
-            uint64_t lm = ballot(parent[k] < 0 && lim[k] < end[k]);
-            while (lm) {
-                const int l = uniform(__builtin_ctzll(lm));
-                lm &= lm - 1;
-                const int bb = readlane(lim[k], l), e = readlane(end[k], l);
-                int hit = -1;
-                for (int r = bb; r < e; r += kWave) {
-                    const int idx = r + lane;
-                    const int uu = idx < e ? g.rcol[idx] : -1;
-                    bool in = false;
-                    if (uu >= 0) {
-                        const unsigned w = use_bitmap ? cur[uu >> 5] : sl.visited[uu >> 5];
-                        const int dl = use_bitmap ? d : sl.depth[uu];
-                        in = ((w >> (uu & 31)) & 1u) && dl == d;
-                    }
-                    const uint64_t hm = ballot(in);
-                    if (hm) {
-                        hit = readlane(uu, uniform(__builtin_ctzll(hm)));
-                        break;
-                    }
-                }
-                if (lane == l) parent[k] = hit;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < G; k++) {
-            const int v = (int) ((b0 + (unsigned) k) * kBlock + threadIdx.x);
-            const bool got = parent[k] >= 0;
-            if (got) {
-                sl.depth[v] = d + 1;
-                // out-degree: the scanned in-row's length when the graph is symmetric
-                o.fe += (unsigned) (g.rrow == g.row ? end[k] - beg[k] : g.row[v + 1] - g.row[v]);
-            }
-            const uint64_t gm = ballot(got);
-            if ((lane & 31) == 0 && valid[k]) {
-                const unsigned bits = (unsigned) (lane == 0 ? gm : gm >> 32);
-                if (bits) sl.visited[(unsigned) v >> 5] = vis[k] | bits;
-                nxt[(unsigned) v >> 5] = bits;  // whole word: also clears it
-            }
-            wq.push(got, v, parent[k]);
-        }
+        wave_lds_sync_b();
     }
 }
 
