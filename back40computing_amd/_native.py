@@ -39,7 +39,8 @@ class BfsConfig(ctypes.Structure):
         ("bu_tile_blocks", ctypes.c_int32),
         ("grab_strided", ctypes.c_int32),
         ("claim_filter", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("spread_steps", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 1),
     ]
 
 

@@ -10,7 +10,9 @@ fastest one whose output is identical to the starting configuration's:
   * tile shape: workgroups per CU, tiles per counter grab, bottom-up tile
     size and grab layout (batched);
   * gather tiers: wave / workgroup / grid row-length thresholds
-    (ExpandByWarp / ExpandByCta, two_phase/expand_atomic/cta.cuh:215-393);
+    (ExpandByWarp / ExpandByCta, two_phase/expand_atomic/cta.cuh:215-393),
+    and whether few-member top-down levels spread their edges evenly over
+    the grid instead;
   * direction switch points: alpha (top-down -> bottom-up) and beta
     (bottom-up -> top-down) of the direction-optimizing search.
 
@@ -43,6 +45,7 @@ SPACE_SINGLE = {
     "grid_gather_threshold": [2048, 4096, 8192, 32768, -1],
     "do_alpha": [2, 4, 8, 14, 24],
     "do_beta": [8, 16, 24, 48],
+    "spread_steps": [1, 2],
 }
 
 SPACE_BATCH = {

@@ -62,7 +62,8 @@ const bfs_config kDefaults = {
     /*bu_tile_blocks*/ 1,
     /*grab_strided*/ 1,
     /*claim_filter*/ 1,
-    {0, 0}};
+    /*spread_steps*/ 1,
+    {0}};
 
 constexpr unsigned kHubCapacity = 1u << 16;  // grid-tier entries per step list
 
@@ -149,6 +150,7 @@ SearchArgs search_args(const bfs_problem *p, int grid)
     s.grid_t = p->cfg.grid_gather_threshold > 0 ? p->cfg.grid_gather_threshold : 0x7fffffff;
     s.grab_max = p->cfg.tile_grab_max > 0 ? p->cfg.tile_grab_max : 1;
     s.coherent_filter = p->cfg.claim_filter == 2;
+    s.spread = p->cfg.spread_steps == 1;
     s.fast_t = p->cfg.fast_path_threshold;
     s.sat_t = p->cfg.saturation_threshold > 0 ? p->cfg.saturation_threshold
               : (p->cfg.saturation_threshold < 0 ? -1 : grid * 256);
@@ -530,6 +532,7 @@ bfs_status bfs_problem_set_config(bfs_problem *p, const bfs_config *cfg)
     if (c.grid_gather_threshold == 0) c.grid_gather_threshold = kDefaults.grid_gather_threshold;
     if (c.grid_gather_threshold < 0) c.grid_gather_threshold = -1;  // grid tier off
     if (c.claim_filter == 0) c.claim_filter = kDefaults.claim_filter;
+    if (c.spread_steps == 0) c.spread_steps = kDefaults.spread_steps;
     if (c.do_alpha == 0) c.do_alpha = kDefaults.do_alpha;
     if (c.do_beta == 0) c.do_beta = kDefaults.do_beta;
     if (c.do_alpha < 1 || c.do_beta < 1) return fail(BFS_ERR_INVALID_ARG, "invalid direction switch points");

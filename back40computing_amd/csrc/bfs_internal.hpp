@@ -49,6 +49,7 @@ struct SearchArgs {
     int instrument;
     int grab_max;          // most tiles taken per tile-counter atomic
     int coherent_filter;   // claim pre-test reads the device-coherent visited word
+    int spread;            // few-member top-down levels split their edges evenly over the grid
     // Search start folded into a persistent launch (init_src >= 0): the
     // kernel clears the status head, queues init_src, labels it
     // (init_label) and, with init_reset, rewrites every label first --

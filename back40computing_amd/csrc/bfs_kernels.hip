@@ -276,10 +276,13 @@ __device__ __forceinline__ void hub_phase(const GraphArgs &g, const SearchArgs &
 
 // ---------------------------------------------------------------------------
 // Step bodies.  All threads of the workgroup execute them together.
-// sub == 0 runs the tile phase, sub == 1 the hub phase.
+// sub == 0 runs the tile phase, sub == 1 the hub phase.  The bodies and the
+// level loop's other parts are force-inlined: one left out of line takes the
+// kernel's argument structs by address, and the compiler then copies them to
+// scratch and reads every field from there for the whole kernel.
 // ---------------------------------------------------------------------------
 template <bool PRED, bool EXACT>
-__device__ void body_expand_contract(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds,
+__device__ __forceinline__ void body_expand_contract(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds,
                                      int sub, bool hubs, unsigned *next_bits = nullptr)
 {
     WaveQueue<PRED> wq = make_queue<PRED>(s, c.step, lds);
@@ -348,7 +351,7 @@ __device__ __forceinline__ void wave_lds_sync()
 }
 
 template <bool PRED, bool EXACT>
-__device__ void body_bottom_up(const GraphArgs &g, const SearchArgs &s, Cursor c, Lds<PRED> &lds)
+__device__ __forceinline__ void body_bottom_up(const GraphArgs &g, const SearchArgs &s, Cursor c, Lds<PRED> &lds)
 {
     WaveQueue<PRED> wq = make_queue<PRED>(s, c.step, lds);
     const unsigned *cur = s.fbm[c.iteration & 1];
@@ -535,7 +538,7 @@ __device__ void body_bottom_up(const GraphArgs &g, const SearchArgs &s, Cursor c
 // is read.  Reservations come from ctl->compact, which the previous step
 // zeroed.
 template <bool PRED>
-__device__ void queue_from_bitmap(const GraphArgs &g, const SearchArgs &s, Cursor c, Lds<PRED> &lds)
+__device__ __forceinline__ void queue_from_bitmap(const GraphArgs &g, const SearchArgs &s, Cursor c, Lds<PRED> &lds)
 {
     unsigned *bm = s.fbm[c.iteration & 1];
     unsigned *other = s.fbm[(c.iteration + 1) & 1];
@@ -756,7 +759,7 @@ __device__ __forceinline__ bool marked_step(const SearchArgs &s, unsigned long l
 }
 
 template <bool PRED, bool EXACT>
-__device__ void body_expand_mark(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds,
+__device__ __forceinline__ void body_expand_mark(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds,
                                  int sub, bool hubs)
 {
     // An edge marks its target when the target's visited bit is clear and,
@@ -895,7 +898,7 @@ __device__ __forceinline__ unsigned byte_bits(unsigned x)
 // (the next step's frontier, exactly: a direction-optimizing search that
 // turns bottom-up next needs no bitmap pass).
 template <bool PRED, bool EXACT>
-__device__ void mark_compact(const GraphArgs &g, const SearchArgs &s, Cursor c, Lds<PRED> &lds, unsigned *fbm_out)
+__device__ __forceinline__ void mark_compact(const GraphArgs &g, const SearchArgs &s, Cursor c, Lds<PRED> &lds, unsigned *fbm_out)
 {
     const unsigned words = ((unsigned) g.nodes + 31) >> 5;
     const int next_depth = c.iteration + 1;
@@ -1130,11 +1133,103 @@ __device__ int fast_scan(FastFront &F, int b, unsigned n)
     return total;
 }
 
+// kFastK consecutive edges from e0 (one thread's share of a batch) of an
+// LDS-resident frontier: offs = the exclusive scan of its n members' degrees
+// (total edges in all), begs = their row starts, vs = the members.  Claims
+// the unvisited neighbours, labels them, and returns the mask of the slots k
+// that claimed u[k] (parent pv[k], row [ub[k], ud[k])).
+template <bool PRED, bool EXACT>
+__device__ __forceinline__ unsigned fast_edges(const GraphArgs &g, const SearchArgs &s, int next_depth,
+                                               const int *offs, const int *begs, const int *vs, unsigned n,
+                                               int total, int e0, int (&u)[kFastK], int (&pv)[kFastK],
+                                               int (&ub)[kFastK], int (&ud)[kFastK])
+{
+    // member holding edge e0: the last i with offs[i] <= e0 (it has edges)
+    int i = 0;
+    if (e0 < total) {
+        int lo = 0, hi = (int) n - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (offs[mid] <= e0) lo = mid;
+            else hi = mid - 1;
+        }
+        i = lo;
+    }
+    // column positions first (LDS only): a loop between two global
+    // loads makes the compiler wait for the first
+    int at[kFastK];
+#pragma unroll
+    for (int k = 0; k < kFastK; k++) {
+        const int e = e0 + k;
+        at[k] = -1;
+        pv[k] = -1;
+        if (e < total) {
+            while (i + 1 < (int) n && offs[i + 1] <= e) i++;
+            at[k] = begs[i] + (e - offs[i]);
+            pv[k] = vs[i];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kFastK; k++) u[k] = at[k] >= 0 ? __builtin_nontemporal_load(g.col + at[k]) : -1;
+    // visited words (a set bit is final), then claims of the clear ones
+    unsigned cur[kFastK];
+#pragma unroll
+    for (int k = 0; k < kFastK; k++) {
+        cur[k] = ~0u;
+        if (u[k] >= 0) {
+            unsigned *w = s.visited + (u[k] >> 5);
+            cur[k] = s.coherent_filter ? __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *w;
+        }
+    }
+    // The claims' results are read only after all are issued (a use
+    // next to its atomic would wait out each round trip in turn).
+    unsigned old[kFastK];
+#pragma unroll
+    for (int k = 0; k < kFastK; k++) {
+        old[k] = ~0u;
+        const unsigned bit = 1u << (u[k] & 31);
+        if (u[k] >= 0 && !(cur[k] & bit))
+            // agent scope: another XCD's L2 may hold this word from
+            // before an earlier step's claims
+            old[k] = __hip_atomic_fetch_or(s.visited + (u[k] >> 5), bit, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // (the empty asm keeps the compiler from folding a use into the
+    // atomic's own block)
+#pragma unroll
+    for (int k = 0; k < kFastK; k++) asm volatile("" : "+v"(old[k]));
+    unsigned fresh = 0;
+#pragma unroll
+    for (int k = 0; k < kFastK; k++)
+        if (!((old[k] >> (u[k] & 31)) & 1u)) fresh |= 1u << k;
+    if (!EXACT) {
+        // best-effort culling elsewhere may have lost bits of labelled vertices
+        int dep[kFastK];
+#pragma unroll
+        for (int k = 0; k < kFastK; k++) dep[k] = ((fresh >> k) & 1) ? s.depth[u[k]] : -1;
+#pragma unroll
+        for (int k = 0; k < kFastK; k++)
+            if (dep[k] != -1) fresh &= ~(1u << k);
+    }
+#pragma unroll
+    for (int k = 0; k < kFastK; k++) {
+        ub[k] = 0;
+        ud[k] = 0;
+        if ((fresh >> k) & 1) {
+            s.depth[u[k]] = next_depth;
+            if (PRED) s.pred[u[k]] = pv[k];
+            ub[k] = g.row[u[k]];
+            ud[k] = g.row[u[k] + 1];
+        }
+    }
+    return fresh;
+}
+
 // One level from LDS buffer src (n members, 0 < n <= kFastCap) into buffer
 // src ^ 1; members past kFastCap go to the global queue of step c + 1 at the
 // same positions.  Every thread calls it.
 template <bool PRED, bool EXACT>
-__device__ void fast_lds_level(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, int src, FastFront &F)
+__device__ __forceinline__ void fast_lds_level(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, int src, FastFront &F)
 {
     const int dst = src ^ 1;
     const int lane = lane_id(), wave = threadIdx.x / kWave;
@@ -1142,93 +1237,12 @@ __device__ void fast_lds_level(const GraphArgs &g, const SearchArgs &s, Cursor c
     int *spill = s.q[(c.step + 1) & 1];
     if (threadIdx.x == 0) F.count = 0;  // ordered before the appends by the scan's barriers
     const int total = fast_scan(F, src, n);
-    const int *offs = F.deg[src];
-    const int *begs = F.beg[src];
-    const int *vs = F.v[src];
     unsigned long long fe = 0;
     bool labeled = false;
     for (int b0 = 0; b0 < total; b0 += kBlock * kFastK) {
-        const int e0 = b0 + (int) threadIdx.x * kFastK;
-        int u[kFastK], pv[kFastK];
-        // member holding edge e0: the last i with offs[i] <= e0 (it has edges)
-        int i = 0;
-        if (e0 < total) {
-            int lo = 0, hi = (int) n - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (offs[mid] <= e0) lo = mid;
-                else hi = mid - 1;
-            }
-            i = lo;
-        }
-        // column positions first (LDS only): a loop between two global
-        // loads makes the compiler wait for the first
-        int at[kFastK];
-#pragma unroll
-        for (int k = 0; k < kFastK; k++) {
-            const int e = e0 + k;
-            at[k] = -1;
-            pv[k] = -1;
-            if (e < total) {
-                while (i + 1 < (int) n && offs[i + 1] <= e) i++;
-                at[k] = begs[i] + (e - offs[i]);
-                pv[k] = vs[i];
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < kFastK; k++) u[k] = at[k] >= 0 ? __builtin_nontemporal_load(g.col + at[k]) : -1;
-        // visited words (a set bit is final), then claims of the clear ones
-        unsigned cur[kFastK];
-#pragma unroll
-        for (int k = 0; k < kFastK; k++) {
-            cur[k] = ~0u;
-            if (u[k] >= 0) {
-                unsigned *w = s.visited + (u[k] >> 5);
-                cur[k] = s.coherent_filter ? __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *w;
-            }
-        }
-        // The claims' results are read only after all are issued (a use
-        // next to its atomic would wait out each round trip in turn).
-        unsigned old[kFastK];
-#pragma unroll
-        for (int k = 0; k < kFastK; k++) {
-            old[k] = ~0u;
-            const unsigned bit = 1u << (u[k] & 31);
-            if (u[k] >= 0 && !(cur[k] & bit))
-                // agent scope: another XCD's L2 may hold this word from
-                // before an earlier step's claims
-                old[k] = __hip_atomic_fetch_or(s.visited + (u[k] >> 5), bit, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-        }
-        // (the empty asm keeps the compiler from folding a use into the
-        // atomic's own block)
-#pragma unroll
-        for (int k = 0; k < kFastK; k++) asm volatile("" : "+v"(old[k]));
-        unsigned fresh = 0;
-#pragma unroll
-        for (int k = 0; k < kFastK; k++)
-            if (!((old[k] >> (u[k] & 31)) & 1u)) fresh |= 1u << k;
-        if (!EXACT) {
-            // best-effort culling elsewhere may have lost bits of labelled vertices
-            int dep[kFastK];
-#pragma unroll
-            for (int k = 0; k < kFastK; k++) dep[k] = ((fresh >> k) & 1) ? s.depth[u[k]] : -1;
-#pragma unroll
-            for (int k = 0; k < kFastK; k++)
-                if (dep[k] != -1) fresh &= ~(1u << k);
-        }
-        int ub[kFastK], ud[kFastK];
-#pragma unroll
-        for (int k = 0; k < kFastK; k++) {
-            ub[k] = 0;
-            ud[k] = 0;
-            if ((fresh >> k) & 1) {
-                s.depth[u[k]] = next_depth;
-                if (PRED) s.pred[u[k]] = pv[k];
-                ub[k] = g.row[u[k]];
-                ud[k] = g.row[u[k] + 1];
-            }
-        }
+        int u[kFastK], pv[kFastK], ub[kFastK], ud[kFastK];
+        const unsigned fresh = fast_edges<PRED, EXACT>(g, s, next_depth, F.deg[src], F.beg[src], F.v[src], n, total,
+                                                       b0 + (int) threadIdx.x * kFastK, u, pv, ub, ud);
         // append the new members: one LDS reservation per wave
         const int cnt = __popc(fresh);
         const int incl = wave_inclusive_sum(cnt);
@@ -1277,6 +1291,100 @@ __device__ void fast_lds_level(const GraphArgs &g, const SearchArgs &s, Cursor c
     __syncthreads();
 }
 
+// Spread step: a top-down level of few members (n <= kFastCap) with more
+// edges than the fast path takes -- typically the first level after the
+// source, a handful of hubs.  The tile phase would hand each row to one
+// workgroup, or publish the long ones for a hub phase behind another grid
+// barrier; here every workgroup loads the whole frontier into LDS, scans its
+// degrees and walks an equal share of the level's edges (kBlock * kFastK at
+// a time, strided over the grid), so the level is one phase and one grid
+// barrier whatever its row lengths.  Each batch reserves its queue slots
+// with one atomic per workgroup.  next_bits as for body_expand_contract.
+// Every thread of every workgroup calls it; F must not be live.
+template <bool PRED, bool EXACT>
+__device__ __forceinline__ void spread_step(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, FastFront &F,
+                            unsigned *next_bits)
+{
+    const int lane = lane_id(), wave = threadIdx.x / kWave;
+    const int *in = s.q[c.step & 1];
+    for (unsigned i = threadIdx.x; i < n; i += kBlock) {
+        const int v = ld_queue(in + i);
+        const int b = g.row[v];
+        F.v[0][i] = v;
+        F.beg[0][i] = b;
+        F.deg[0][i] = g.row[v + 1] - b;
+    }
+    __syncthreads();
+    const int total = fast_scan(F, 0, n);
+    const int next_depth = c.iteration + 1;
+    Control *ctl = s.ctl;
+    int *out = s.q[(c.step + 1) & 1];
+    int *pout = PRED ? s.pq[(c.step + 1) & 1] : nullptr;
+    unsigned long long fe = 0;
+    bool labeled = false;
+    const int stride = (int) gridDim.x * kBlock * kFastK;
+    for (int b0 = (int) blockIdx.x * kBlock * kFastK; b0 < total; b0 += stride) {
+        int u[kFastK], pv[kFastK], ub[kFastK], ud[kFastK];
+        const unsigned fresh = fast_edges<PRED, EXACT>(g, s, next_depth, F.deg[0], F.beg[0], F.v[0], n, total,
+                                                       b0 + (int) threadIdx.x * kFastK, u, pv, ub, ud);
+        int got;
+        const int off = block_exclusive_sum(__popc(fresh), F.scan, &got);
+        // (every thread read the previous batch's base before the scan's
+        // first barrier)
+        if (threadIdx.x == 0 && got) F.count = add_agent(&ctl->qlen[(c.step + 1) % 3], (unsigned) got);
+        __syncthreads();
+        const unsigned p0 = F.count + (unsigned) off;
+        bool lost = false;
+#pragma unroll
+        for (int k = 0; k < kFastK; k++) {
+            const unsigned p = p0 + (unsigned) __popc(fresh & ((1u << k) - 1u));
+            if ((fresh >> k) & 1) {
+                fe += (unsigned) (ud[k] - ub[k]);
+                if (p < s.cap) {
+                    out[p] = u[k];
+                    if (PRED) pout[p] = pv[k];
+                } else {
+                    lost = true;
+                }
+            }
+        }
+        labeled |= fresh != 0;
+        if (lost) {
+            st_agent(&ctl->overflow, 1u);
+            st_agent(&ctl->ovf[(c.step + 1) % 3], 1u);
+        }
+        if (next_bits) {
+#pragma unroll
+            for (int k = 0; k < kFastK; k++)
+                if ((fresh >> k) & 1)
+                    __hip_atomic_fetch_or(next_bits + (u[k] >> 5), 1u << (u[k] & 31), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    // one update per workgroup of the next frontier's edges and the depth
+    const int lo = wave_inclusive_sum((int) (fe & 0xFFFFu));
+    const int hi = wave_inclusive_sum((int) (fe >> 16));
+    const bool any = ballot(labeled) != 0;
+    if (lane == kWave - 1) {
+        F.wfe[wave] = ((unsigned long long) (unsigned) hi << 16) + (unsigned long long) (unsigned) lo;
+        F.wlab[wave] = any ? 1 : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long f = 0;
+        int l = 0;
+        for (int w = 0; w < kWaves; w++) {
+            f += F.wfe[w];
+            l |= F.wlab[w];
+        }
+        if (s.track_edges && f) {
+            __hip_atomic_fetch_add(&ctl->fedges[(c.step + 1) % 3], f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&ctl->visited_edges, f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (l) atomicMax(&ctl->max_depth, next_depth);
+    }
+}
+
 // Runs LDS fast levels from step c (input: n <= kFastCap members in the
 // global queue) until a level's output is empty, too large for the path, or
 // (direction-optimizing) calls for bottom-up.  Leaves the global queue,
@@ -1284,7 +1392,7 @@ __device__ void fast_lds_level(const GraphArgs &g, const SearchArgs &s, Cursor c
 // `fv`: visited-edge sum through step c (see choose_direction).  Block 0
 // only; every thread calls it.
 template <bool PRED, bool EXACT>
-__device__ void fast_lds_path(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, int loop,
+__device__ __forceinline__ void fast_lds_path(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, int loop,
                               unsigned long long fv, FastFront &F, bool wait_others)
 {
     const int *in = s.q[c.step & 1];
@@ -1372,7 +1480,7 @@ __device__ void fast_lds_path(const GraphArgs &g, const SearchArgs &s, Cursor c,
 // before a grid barrier as long as it reads nothing of the control block
 // and claims nothing before one (the loop top takes step 0's values from
 // search_start_values).
-__device__ void search_prologue(const GraphArgs &g, const SearchArgs &s, bool clear_bitmaps)
+__device__ __forceinline__ void search_prologue(const GraphArgs &g, const SearchArgs &s, bool clear_bitmaps)
 {
     const int src = s.init_src;
     if (clear_bitmaps) {
@@ -1423,7 +1531,7 @@ __device__ void search_prologue(const GraphArgs &g, const SearchArgs &s, bool cl
 // step's counter updates, and the exit fields are block 0's own: the head
 // is final (the instrumented cycle sums, added later, are not copied).
 // Called by thread 0 of block 0.
-__device__ void publish_status(const SearchArgs &s)
+__device__ __forceinline__ void publish_status(const SearchArgs &s)
 {
     const unsigned *head = reinterpret_cast<const unsigned *>(s.ctl);
     unsigned *out = reinterpret_cast<unsigned *>(s.host_head);
@@ -1584,9 +1692,12 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
 
         unsigned long long t0 = s.instrument ? wall_clock64() : 0;
         // large top-down levels mark instead of claiming (grid-uniform: fe
-        // is this step's frontier-edge count)
+        // is this step's frontier-edge count); few-member ones spread their
+        // edges over the grid
         const bool mark =
             (LOOP == kLoopEC || LOOP == kLoopDO) && c.phase == kPhaseExpandContract && marked_step(s, fe);
+        const bool spread = (LOOP == kLoopEC || LOOP == kLoopDO) && c.phase == kPhaseExpandContract && !mark &&
+                            n <= (unsigned) kFastCap && s.spread;
         // A top-down step of a direction-optimizing search marks its claims
         // in the next frontier's bitmap when that bitmap is clear, so a turn
         // to bottom-up after it needs no bitmap pass.
@@ -1598,11 +1709,20 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
             fbm_exact_step = c.step + 1;
         }
         if (blockIdx.x == 0 && threadIdx.x == 0) prepare_step(s.ctl, c, n);
-        if (mark) body_expand_mark<PRED, EXACT>(g, s, c, n, lds, 0, true);
-        else loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 0, true, next_bits);
+        if (spread) {
+            if constexpr (LOOP == kLoopEC || LOOP == kLoopDO) {
+                spread_step<PRED, EXACT>(g, s, c, n, shared.fast, next_bits);
+                __syncthreads();
+                init_lds(lds);
+            }
+        } else if (mark) {
+            body_expand_mark<PRED, EXACT>(g, s, c, n, lds, 0, true);
+        } else {
+            loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 0, true, next_bits);
+        }
         if (s.instrument) busy += wall_clock64() - t0;
         grid_barrier(s.ctl);
-        if (gathers(c.phase) && hubs_pending(s, c.step)) {
+        if (!spread && gathers(c.phase) && hubs_pending(s, c.step)) {
             t0 = s.instrument ? wall_clock64() : 0;
             if (mark) body_expand_mark<PRED, EXACT>(g, s, c, n, lds, 1, true);
             else loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 1, true, next_bits);

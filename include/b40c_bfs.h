@@ -117,7 +117,11 @@ typedef struct bfs_config {
     int32_t claim_filter;           /* exact claims: the visited-word test before the
                                        atomic reads 1 = the cached copy, 2 = the
                                        device-coherent value */
-    int32_t reserved[2];
+    int32_t spread_steps;           /* top-down levels of at most 1024 vertices with
+                                       too many edges for one workgroup: 1 = every
+                                       workgroup walks an equal share of their edges
+                                       (one phase), 2 = tile and grid-tier phases */
+    int32_t reserved[1];
 } bfs_config;
 
 typedef struct bfs_stats {

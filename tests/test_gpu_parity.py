@@ -209,6 +209,62 @@ def test_marked_top_down_levels(gpu, cull, mark_pred):
     p.destroy()
 
 
+def hub_fan_graph(hubs=3, leaves=20000, seed=5):
+    """Source 0 -> hubs 1..hubs, each hub -> its own leaves, and a random
+    sparse mesh over the leaves: level 1 has a few members with tens of
+    thousands of edges (more than one workgroup's batch, fewer than a
+    marked level's), the shape of an R-MAT search's first levels."""
+    rng = np.random.default_rng(seed)
+    n = 1 + hubs + hubs * leaves
+    src = [np.zeros(hubs, np.int64)]
+    dst = [np.arange(1, hubs + 1)]
+    for h in range(hubs):
+        lv = 1 + hubs + h * leaves + np.arange(leaves)
+        src.append(np.full(leaves, 1 + h))
+        dst.append(lv)
+    a = rng.integers(1 + hubs, n, 2 * leaves)
+    b = rng.integers(1 + hubs, n, 2 * leaves)
+    src.append(a)
+    dst.append(b)
+    return CsrGraph.from_coo(n, np.concatenate(src), np.concatenate(dst), symmetrize=True, dedup=True,
+                             drop_self_loops=True)
+
+
+@pytest.mark.parametrize("cull", [b40c.CULL_EXACT, b40c.CULL_BEST_EFFORT], ids=["exact", "best_effort"])
+@pytest.mark.parametrize("mark_pred", [False, True], ids=["depth", "pred"])
+def test_spread_levels(gpu, cull, mark_pred):
+    # Top-down levels of at most 1024 members whose edges exceed one
+    # workgroup's batch spread the edges evenly over the grid (spread_steps
+    # 1, the default) instead of tile + grid-tier phases (2).  Both forms,
+    # both loops, on the hub fan and on R-MAT (whose first levels have it).
+    graphs = [hub_fan_graph(), CsrGraph.rmat(16, 16, seed=0x5B)]
+    for g in graphs:
+        p = CsrProblem(mark_predecessors=mark_pred).from_host_problem(g)
+        # best-effort culling queues duplicates (at most a level's edges):
+        # a queue of E + V entries cannot overflow
+        qs = g.edges / g.nodes + 1.0
+        for spread in (1, 2):
+            p.set_config(cull_mode=cull, spread_steps=spread)
+            # (source 1 is a hub: the first step of the search spreads)
+            for src in [0, 1] if g.nodes < 70000 else g.sample_sources(2, seed=3):
+                assert_parity(g, p, b40c.EXPAND_CONTRACT, True, src, queue_sizing=qs)
+                assert_parity(g, p, b40c.DIRECTION_OPTIMIZING, True, src, queue_sizing=qs)
+                p.set_config(do_alpha=1 << 20)
+                assert_parity(g, p, b40c.DIRECTION_OPTIMIZING, True, src, queue_sizing=qs)
+                p.set_config(do_alpha=8)
+        p.destroy()
+    # a spread level that overflows the queue reports it, and the problem recovers
+    g = graphs[0]
+    p = CsrProblem(mark_predecessors=mark_pred).from_host_problem(g)
+    p.set_config(cull_mode=cull)
+    p.reset(b40c.VERTEX_FRONTIERS, 0.5)
+    with pytest.raises(b40c.BfsError) as ei:
+        p.enact(b40c.EXPAND_CONTRACT, 0)
+    assert ei.value.status == 3  # BFS_ERR_OVERFLOW
+    assert_parity(g, p, b40c.EXPAND_CONTRACT, True, 0)
+    p.destroy()
+
+
 def test_invalid_source_and_frontier_type(gpu):
     g = CsrGraph.grid2d(4)
     p = CsrProblem().from_host_problem(g)
