@@ -283,7 +283,7 @@ __device__ __forceinline__ void hub_phase(const GraphArgs &g, const SearchArgs &
 // ---------------------------------------------------------------------------
 template <bool PRED, bool EXACT>
 __device__ __forceinline__ void body_expand_contract(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds,
-                                     int sub, bool hubs, unsigned *next_bits = nullptr)
+                                     int sub, bool hubs, unsigned *next_bits = nullptr, bool from_bitmap = false)
 {
     WaveQueue<PRED> wq = make_queue<PRED>(s, c.step, lds);
     const int *in = s.q[c.step & 1];
@@ -303,7 +303,37 @@ __device__ __forceinline__ void body_expand_contract(const GraphArgs &g, const S
         }
         wq.push(fresh, u, par);
     };
-    if (sub == 0) {
+    if (sub == 0 && from_bitmap) {
+        // The frontier is frontier bitmap fbm[c.iteration & 1], as a
+        // bottom-up level left it: each workgroup takes kBlock-word tiles
+        // of it, clears them and the other bitmap's words (so both are
+        // clear after the step, as a top-down level needs them), and
+        // gathers one member per thread per round.
+        const HubList hub = hub_list(s, c.step, hubs, n);
+        unsigned *bm = s.fbm[c.iteration & 1];
+        unsigned *other = s.fbm[(c.iteration + 1) & 1];
+        const unsigned words = ((unsigned) g.nodes + 31) >> 5;
+        for (unsigned w0 = blockIdx.x * kBlock; w0 < words; w0 += gridDim.x * kBlock) {
+            const unsigned w = w0 + threadIdx.x;
+            unsigned bits = 0u;
+            if (w < words) {
+                bits = bm[w];
+                bm[w] = 0u;
+                other[w] = 0u;
+            }
+            while (__syncthreads_or(bits != 0u)) {
+                int beg = 0, end = 0, par = -1;
+                if (bits) {
+                    const int v = (int) ((w << 5) + (unsigned) __builtin_ctz(bits));
+                    bits &= bits - 1u;
+                    beg = g.row[v];
+                    end = g.row[v + 1];
+                    par = v;
+                }
+                gather_rows<PRED>(beg, end, par, g.col, s.wave_t, s.cta_t, hub, lds, edge);
+            }
+        }
+    } else if (sub == 0) {
         tile_phase<PRED>(g, s, c, n, hub_list(s, c.step, hubs, n), lds, [&](unsigned i, int &beg, int &end, int &par) {
             int v = ld_queue(in + i);
             beg = g.row[v];
@@ -696,7 +726,8 @@ __device__ __forceinline__ void run_body(int phase, const GraphArgs &g, const Se
 // its kernel, which keeps their registers out of its budget).
 template <bool PRED, bool EXACT, int LOOP>
 __device__ __forceinline__ void loop_body(int phase, const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n,
-                                          Lds<PRED> &lds, int sub, bool hubs, unsigned *next_bits = nullptr)
+                                          Lds<PRED> &lds, int sub, bool hubs, unsigned *next_bits = nullptr,
+                                          bool from_bitmap = false)
 {
     if constexpr (LOOP == kLoopEC) {
         body_expand_contract<PRED, EXACT>(g, s, c, n, lds, sub, hubs);
@@ -704,7 +735,7 @@ __device__ __forceinline__ void loop_body(int phase, const GraphArgs &g, const S
         body_contract_expand<PRED, EXACT>(g, s, c, n, lds, sub, hubs);
     } else if constexpr (LOOP == kLoopDO) {
         if (phase == kPhaseBottomUp) body_bottom_up<PRED, EXACT>(g, s, c, lds);
-        else body_expand_contract<PRED, EXACT>(g, s, c, n, lds, sub, hubs, next_bits);
+        else body_expand_contract<PRED, EXACT>(g, s, c, n, lds, sub, hubs, next_bits, from_bitmap);
     } else {
         if (phase == kPhaseContract) body_contract<PRED, EXACT>(g, s, c, n, lds);
         else if (phase == kPhaseExpand) body_expand<PRED, EXACT>(g, s, c, n, lds, sub, hubs);
@@ -1581,6 +1612,7 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
     while (true) {
         unsigned n;
         unsigned long long fe = 0;
+        bool bu_exit = false;  // bottom-up until this step
         if (start) {
             n = 1;
             if (s.track_edges && s.init_label) fe = (unsigned long long) (g.row[s.init_src + 1] - g.row[s.init_src]);
@@ -1612,15 +1644,29 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
                 }
                 fbm_dirty = 3u;  // bottom-up levels write both
             } else if (dir != kPhaseBottomUp && c.phase == kPhaseBottomUp) {
-                queue_from_bitmap<PRED>(g, s, c, lds);  // clears both bitmaps
-                grid_barrier(s.ctl);
-                fbm_dirty = 0u;
+                bu_exit = true;
             }
             c.phase = dir;
         }
 
         const bool fast = fast_level(s, c, n, fe) && gridDim.x > 1 &&
                           ((LOOP != kLoopEC && LOOP != kLoopDO) || (n <= (unsigned) kFastCap && fe <= kFastEdges));
+        // large top-down levels mark instead of claiming (grid-uniform: fe
+        // is this step's frontier-edge count); few-member ones spread their
+        // edges over the grid
+        const bool mark =
+            (LOOP == kLoopEC || LOOP == kLoopDO) && c.phase == kPhaseExpandContract && marked_step(s, fe);
+        const bool spread = (LOOP == kLoopEC || LOOP == kLoopDO) && c.phase == kPhaseExpandContract && !mark &&
+                            n <= (unsigned) kFastCap && s.spread;
+        // The first top-down step after bottom-up levels reads its frontier
+        // from the bitmap they left, unless its form needs the queue (the
+        // fast path, spread and marked steps index it).
+        const bool from_bitmap = bu_exit && !fast && !mark && !spread;
+        if (bu_exit && !from_bitmap) {
+            queue_from_bitmap<PRED>(g, s, c, lds);  // clears both bitmaps
+            grid_barrier(s.ctl);
+            fbm_dirty = 0u;
+        }
         if (start && !fast) grid_barrier(s.ctl);
         const bool first = start;
         start = false;
@@ -1691,13 +1737,6 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
         }
 
         unsigned long long t0 = s.instrument ? wall_clock64() : 0;
-        // large top-down levels mark instead of claiming (grid-uniform: fe
-        // is this step's frontier-edge count); few-member ones spread their
-        // edges over the grid
-        const bool mark =
-            (LOOP == kLoopEC || LOOP == kLoopDO) && c.phase == kPhaseExpandContract && marked_step(s, fe);
-        const bool spread = (LOOP == kLoopEC || LOOP == kLoopDO) && c.phase == kPhaseExpandContract && !mark &&
-                            n <= (unsigned) kFastCap && s.spread;
         // A top-down step of a direction-optimizing search marks its claims
         // in the next frontier's bitmap when that bitmap is clear, so a turn
         // to bottom-up after it needs no bitmap pass.
@@ -1718,14 +1757,14 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
         } else if (mark) {
             body_expand_mark<PRED, EXACT>(g, s, c, n, lds, 0, true);
         } else {
-            loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 0, true, next_bits);
+            loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 0, true, next_bits, from_bitmap);
         }
         if (s.instrument) busy += wall_clock64() - t0;
         grid_barrier(s.ctl);
         if (!spread && gathers(c.phase) && hubs_pending(s, c.step)) {
             t0 = s.instrument ? wall_clock64() : 0;
             if (mark) body_expand_mark<PRED, EXACT>(g, s, c, n, lds, 1, true);
-            else loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 1, true, next_bits);
+            else loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 1, true, next_bits, from_bitmap);
             if (s.instrument) busy += wall_clock64() - t0;
             grid_barrier(s.ctl);
         }
@@ -1741,6 +1780,8 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
             if (s.instrument) busy += wall_clock64() - t0;
             grid_barrier(s.ctl);
         }
+        // (fbm_dirty kept this step's claims out of the bitmap it cleared)
+        if (from_bitmap) fbm_dirty = 0u;
         c = advance(loop, c);
     }
 

@@ -159,6 +159,24 @@ def test_direction_switch_points(gpu, fused):
         p.destroy()
 
 
+@pytest.mark.parametrize("cull", [b40c.CULL_EXACT, b40c.CULL_BEST_EFFORT], ids=["exact", "best_effort"])
+@pytest.mark.parametrize("mark_pred", [False, True], ids=["depth", "pred"])
+def test_top_down_after_bottom_up(gpu, cull, mark_pred):
+    # The first top-down level after bottom-up ones reads its frontier from
+    # the bottom-up bitmap when it is an ordinary step (more than 1024
+    # members, fewer than a marked level's edges), else from a queue built
+    # from the bitmap.  beta = 1 turns back after every bottom-up level.
+    g = CsrGraph.rmat(15, 16, seed=0x3C)
+    p = CsrProblem(mark_predecessors=mark_pred).from_host_problem(g)
+    qs = g.edges / g.nodes + 1.0
+    for alpha, beta in [(1000000, 1), (2, 4), (8, 48)]:
+        for spread in (1, 2):
+            p.set_config(cull_mode=cull, do_alpha=alpha, do_beta=beta, spread_steps=spread)
+            for src in g.sample_sources(3, seed=11):
+                assert_parity(g, p, b40c.DIRECTION_OPTIMIZING, True, src, queue_sizing=qs)
+    p.destroy()
+
+
 def test_grid_lattices_small(gpu):
     for w, builder in [(37, CsrGraph.grid2d), (11, CsrGraph.grid3d)]:
         g = builder(w)
