@@ -124,9 +124,11 @@ def test_gather_tiers_and_grid_shapes(gpu, variant):
     for wave_t, cta_t, grid_t, fast_t, grid in [(1, 2, 3, -1, 0), (2, 5, 40, 64, 3), (64, 65, 100, -1, 1),
                                                 (1000000, 1000000, -1, 16, 0), (8, 4096, 2049, 1 << 30, 0),
                                                 (64, 1024, 8192, 1024, 0)]:
-        p.set_config(wave_gather_threshold=wave_t, cta_gather_threshold=cta_t,
-                     grid_gather_threshold=grid_t, fast_path_threshold=fast_t)
-        assert_parity(g, p, strategy, fused, src, max_grid_size=grid)
+        # (spread steps off too: small levels then take the tile phases)
+        for spread in (1, 2):
+            p.set_config(wave_gather_threshold=wave_t, cta_gather_threshold=cta_t,
+                         grid_gather_threshold=grid_t, fast_path_threshold=fast_t, spread_steps=spread)
+            assert_parity(g, p, strategy, fused, src, max_grid_size=grid)
     p.destroy()
 
 
@@ -174,6 +176,11 @@ def test_top_down_after_bottom_up(gpu, cull, mark_pred):
             p.set_config(cull_mode=cull, do_alpha=alpha, do_beta=beta, spread_steps=spread)
             for src in g.sample_sources(3, seed=11):
                 assert_parity(g, p, b40c.DIRECTION_OPTIMIZING, True, src, queue_sizing=qs)
+    # tiny grids: each workgroup walks many bitmap tiles and spread batches
+    p.set_config(do_alpha=1000000, do_beta=1, spread_steps=1)
+    for grid in (1, 3):
+        for src in g.sample_sources(2, seed=12):
+            assert_parity(g, p, b40c.DIRECTION_OPTIMIZING, True, src, queue_sizing=qs, max_grid_size=grid)
     p.destroy()
 
 
