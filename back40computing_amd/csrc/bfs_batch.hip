@@ -266,10 +266,10 @@ bfs_status bfs_batch_config_default(bfs_config *cfg)
     bfs_config_default(cfg);
     // Autotuned on the 256-query R-MAT pool (64 graphs, scale 14..20):
     // bottom-up tiles of up to 64 blocks (16384 consecutive vertices per
-    // workgroup; levels with few blocks use smaller ones), four workgroups
-    // per CU (capped at half the occupancy), large consecutive grabs and an
-    // early grid tier (rows of 512 neighbours and more).
-    cfg->blocks_per_cu = 4;
+    // workgroup; levels with few blocks use smaller ones), two workgroups
+    // per CU, large consecutive grabs and an early grid tier (rows of 512
+    // neighbours and more).
+    cfg->blocks_per_cu = 2;
     cfg->bu_tile_blocks = 64;
     cfg->tile_grab_max = 128;
     cfg->grab_strided = 1;
