@@ -56,8 +56,8 @@ SPACE_BATCH = {
     "grab_strided": [1, 2],
     "wave_gather_threshold": [32, 64, 128],
     "cta_gather_threshold": [256, 512, 1024],
-    "grid_gather_threshold": [512, 1024, 4096],
-    "do_alpha": [2, 4, 8],
+    "grid_gather_threshold": [256, 512, 1024, 4096],
+    "do_alpha": [4, 8, 12],
     "do_beta": [8, 24, 48],
 }
 

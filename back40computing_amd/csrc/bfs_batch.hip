@@ -267,16 +267,17 @@ bfs_status bfs_batch_config_default(bfs_config *cfg)
     // Autotuned on the 256-query R-MAT pool (64 graphs, scale 14..20):
     // bottom-up tiles of up to 64 blocks (16384 consecutive vertices per
     // workgroup; levels with few blocks use smaller ones), two workgroups
-    // per CU, large consecutive grabs and an early grid tier (rows of 512
-    // neighbours and more).
+    // per CU, large consecutive grabs, an early grid tier (rows of 256
+    // neighbours and more) and a later turn to bottom-up than one search
+    // alone takes (alpha 12).
     cfg->blocks_per_cu = 2;
     cfg->bu_tile_blocks = 64;
     cfg->tile_grab_max = 128;
     cfg->grab_strided = 1;
-    cfg->grid_gather_threshold = 512;
+    cfg->grid_gather_threshold = 256;
     cfg->wave_gather_threshold = 64;
-    cfg->cta_gather_threshold = 1024;
-    cfg->do_alpha = 8;
+    cfg->cta_gather_threshold = 512;
+    cfg->do_alpha = 12;
     cfg->do_beta = 48;
     return BFS_OK;
 }
