@@ -228,8 +228,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this BFS build runs on a single MI355X "
-              "(see DESIGN.md for how the search would shard); got --gpus %d" % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU only: this BFS library runs a search on a single MI355X "
+              "(sharding a search across GPUs is not built); got --gpus %d" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

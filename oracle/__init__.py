@@ -2,7 +2,8 @@
 
 Loads oracle/liboracle_bfs.so (built from oracle/bfs_oracle.c by
 ``make -C oracle``).  Only tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg may use it; it is the checker, never the product.
+cpu_baseline leg may use it; results are compared against it, and the
+product never calls it.
 """
 import ctypes
 import os

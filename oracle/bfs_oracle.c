@@ -2,7 +2,7 @@
  * bfs_oracle.c -- CPU restatement of b40c's serial reference BFS and of the
  * test driver's scoring and validation.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP traversal
+ * TEST INFRASTRUCTURE ONLY.  This file is the reference the HIP traversal
  * is compared against.  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load it; the product library never links
  * or calls it.

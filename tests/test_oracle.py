@@ -169,7 +169,7 @@ def test_oracle_rmat_invariants():
 # ---------------------------------------------------------------------------
 # Predecessor validation and comparison
 # ---------------------------------------------------------------------------
-def test_oracle_predecessors_valid_and_checker_rejects_corruptions():
+def test_oracle_predecessors_valid_and_validation_rejects_corruptions():
     g = CsrGraph.rmat(10, 8, seed=11)
     row, col = g.row_offsets.copy(), g.column_indices.copy()
     s = int(g.sample_sources(1, seed=5)[0])
