@@ -9,6 +9,16 @@ from . import _native
 from .bfs import BfsError, _check
 
 
+def _destroy_quietly(obj):
+    """destroy() from __del__: a partly built object, or an interpreter
+    already tearing the module down, must not raise there."""
+    try:
+        if getattr(obj, "_h", None):
+            obj.destroy()
+    except Exception:
+        pass
+
+
 class GraphPool:
     """Device-resident graphs shared by batches and servers."""
 
@@ -35,6 +45,9 @@ class GraphPool:
         if self._h:
             _native.load().bfs_pool_destroy(self._h)
             self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        _destroy_quietly(self)
 
 
 def query_state_bytes(nodes):
@@ -94,6 +107,9 @@ class Batch:
         if self._h:
             _native.load().bfs_batch_destroy(self._h)
             self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        _destroy_quietly(self)
 
 
 class _PinnedBlock:
@@ -178,6 +194,13 @@ class Server:
         return {f: getattr(st, f) for f, _ in st._fields_}
 
     def destroy(self):
+        """Finishes resident queries and joins the server threads; the
+        result arrays of tickets not yet waited for stay valid after it."""
         if self._h:
             _native.load().bfs_server_destroy(self._h)
             self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        # the native threads may still write into the pinned result arrays
+        # of unfinished tickets: join them before those arrays can be freed
+        _destroy_quietly(self)

@@ -177,7 +177,7 @@ struct SlotTable {
         return a;
     }
 
-    // Captures `levels` (level kernel, advance kernel) pairs into a graph.
+    // Captures `levels` batch levels (level, hub and advance kernels) into a graph.
     bfs_status capture(const BatchArgs &a, int grid, int levels)
     {
         // the kernel arguments are baked into the graph: any change re-captures
@@ -488,7 +488,7 @@ bfs_status bfs_batch_enact(bfs_batch *b, const int32_t *graph_ids, const int32_t
             while (true) {
                 hipError_t e = hipGraphLaunch(t.graph_exec, t.stream);
                 if (e != hipSuccess) { st = host_hip_fail(e, "launch level graph"); break; }
-                launches += 2 * levels;
+                launches += kBatchLevelLaunches * levels;
                 e = hipMemcpyAsync(t.h_alive, t.d_alive, sizeof(int), hipMemcpyDeviceToHost, t.stream);
                 if (e == hipSuccess) e = hipStreamSynchronize(t.stream);
                 if (e != hipSuccess) { st = host_hip_fail(e, "read alive count"); break; }
@@ -931,12 +931,19 @@ bfs_status bfs_server_wait(bfs_server *s, int64_t ticket, int32_t timeout_ms, bf
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->reqs.find(ticket);
     if (it == s->reqs.end()) return host_fail(BFS_ERR_INVALID_ARG, "unknown ticket");
-    auto ready = [&] { return s->reqs[ticket].done; };
+    // A ticket another waiter has already collected counts as finished (no
+    // entry is ever re-created here); this waiter then reports it unknown.
+    auto ready = [&] {
+        auto f = s->reqs.find(ticket);
+        return f == s->reqs.end() || f->second.done;
+    };
     if (timeout_ms < 0) s->cv_done.wait(lk, ready);
     else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
         return host_fail(BFS_ERR_TIMEOUT, "wait timed out");
-    Request r = s->reqs[ticket];
-    s->reqs.erase(ticket);
+    it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return host_fail(BFS_ERR_INVALID_ARG, "ticket collected by another waiter");
+    Request r = it->second;
+    s->reqs.erase(it);
     if (info) {
         info->ticket = r.ticket;
         info->graph_id = r.graph;

@@ -9,8 +9,10 @@
 namespace b40c_hip {
 
 hipError_t launch_batch_persistent(const BatchArgs &a, int grid, hipStream_t st);
-// One level (all queries) followed by the single-workgroup advance kernel;
+// One level (all queries): the level kernel, the grid-tier (hub) kernel and
+// the single-workgroup advance kernel, kBatchLevelLaunches launches in all;
 // *alive_out (device) receives the number of queries still running.
+constexpr int kBatchLevelLaunches = 3;
 hipError_t launch_batch_level(const BatchArgs &a, int grid, int *alive_out, hipStream_t st);
 // Resets and seeds the queries whose slot indices are in d_list[0..count).
 hipError_t launch_batch_seed(const BatchArgs &a, const int *d_list, int count, int max_nodes, hipStream_t st);

@@ -743,6 +743,10 @@ bfs_status bfs_barrier_benchmark(int32_t device, int32_t blocks_per_cu, int32_t 
     HIP_TRY(hipSetDevice(device), "hipSetDevice");
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "query CU count");
+    // the co-residency rule of every persistent launch (persistent_grid)
+    const int occ = barrier_bench_max_blocks_per_cu();
+    if (blocks_per_cu > std::max(1, std::min(kMaxPersistentPerCu, occ / 2)))
+        return fail(BFS_ERR_INVALID_ARG, "blocks_per_cu above the persistent-grid cap min(4, occupancy / 2)");
     int grid = cus * blocks_per_cu;
     if (grid_size) *grid_size = grid;
     Control *d_ctl = nullptr;

@@ -529,11 +529,21 @@ __device__ __forceinline__ void grid_barrier(Control *ctl)
         if (arrived == leaf_size - 1) {
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
             __hip_atomic_store(lc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // Relaxed stores to different lines may land in any order, and a
+            // relaxed RMW does not wait for them: the reset must be visible
+            // before the root arrival that can release this leaf's
+            // workgroups, or their next arrival could be overwritten by it.
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             unsigned r = __hip_atomic_fetch_add(&ctl->bar_root, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (r == leaves - 1) {
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
                 __hip_atomic_store(&ctl->bar_root, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 __hip_atomic_store(&ctl->bar_gen, gen + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                // the new generation and the root reset before any release
+                // word: a released workgroup reads bar_gen at its next
+                // barrier, and an old value there would let it straight
+                // through
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
                 for (unsigned l = 0; l < leaves; l++)
                     __hip_atomic_store(&ctl->bar_release[l * 32], gen + 1, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);

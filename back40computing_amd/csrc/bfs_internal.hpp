@@ -92,6 +92,7 @@ hipError_t launch_persistent(int loop, bool pred, bool exact, const GraphArgs &g
 // Largest co-resident grid of the persistent kernel for `loop` (0 on error).
 int persistent_max_blocks_per_cu(int loop, bool pred, bool exact);
 int step_max_blocks_per_cu(bool pred);
+int barrier_bench_max_blocks_per_cu();
 hipError_t launch_barrier_bench(Control *ctl, int iterations, int grid, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st);
 hipError_t launch_wave_selftest(const int *in, int *out_scan, int *out_rank, int n, hipStream_t st);

@@ -2052,6 +2052,15 @@ int step_max_blocks_per_cu(bool pred)
     return blocks;
 }
 
+int barrier_bench_max_blocks_per_cu()
+{
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void *>(&barrier_bench_kernel),
+                                                     kBlock, 0) != hipSuccess)
+        return 0;
+    return blocks;
+}
+
 hipError_t launch_barrier_bench(Control *ctl, int iterations, int grid, hipStream_t st)
 {
     Control *c = ctl;

@@ -1,9 +1,9 @@
 #!/usr/bin/env python3
 """BFS throughput on one MI355X: R-MAT scale 22, edge factor 16, 64 sources.
 
-A step is one single-source search (source k of the 64 seeded sources, in
-turn) over the device-resident graph: label reset + traversal through the
-C ABI of libb40c_hip.so.  value = total traversed edges (E_vis, counted as
+A step is one sweep over the 64 seeded sources: for each, a single-source
+search over the device-resident graph (label reset + traversal through the
+C ABI of libb40c_hip.so).  value = total traversed edges (E_vis, counted as
 test/bfs/test_bfs.cu:360-367) / wall time of the K timed steps.  The default
 strategy is direction-optimizing; the top-down expand-contract rate over the
 same sources is reported beside it.
@@ -206,8 +206,8 @@ def bench_serving(b40c, np, rmat_scale, rates, queries_per_rate):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64)
-    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=10, help="timed sweeps over all sources")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed sweeps")
     ap.add_argument("--strategy", choices=sorted(STRATEGIES), default="do")
     ap.add_argument("--iterative", action="store_true", help="per-level launches instead of the persistent kernel")
     ap.add_argument("--scale", type=int, default=22)
@@ -276,22 +276,25 @@ def main():
             kept[i] = d
     log("per-source E_vis: min %d max %d" % (e_vis.min(), e_vis.max()))
 
+    # A step is one sweep over every sampled source, so the timed region
+    # covers all of them whatever --steps is.
     for k in range(args.warmup):
-        p.enact(strategy, int(srcs[k % len(srcs)]), fused=fused)
+        for s in srcs:
+            p.enact(strategy, int(s), fused=fused)
 
     lib.bfs_device_synchronize(0)
-    kernel_ms = np.zeros(args.steps)
+    kernel_ms = np.zeros((args.steps, len(srcs)))
     edges = 0
     t0 = time.perf_counter()
     for k in range(args.steps):
-        i = k % len(srcs)
-        st = p.enact(strategy, int(srcs[i]), fused=fused)
-        kernel_ms[k] = st.elapsed_ms
-        edges += int(e_vis[i])
+        for i, s in enumerate(srcs):
+            kernel_ms[k, i] = p.enact(strategy, int(s), fused=fused).elapsed_ms
+        edges += int(e_vis.sum())
     lib.bfs_device_synchronize(0)
     wall = time.perf_counter() - t0
 
-    idx = np.arange(args.steps) % len(srcs)
+    kernel_ms = kernel_ms.ravel()
+    idx = np.tile(np.arange(len(srcs)), args.steps)
     gteps = edges / wall / 1e9
     per_src = e_vis[idx] / (kernel_ms * 1e-3) / 1e9
     b_alg = 8.0 * e_vis[idx] + 20.0 * v_vis[idx]
