@@ -8,5 +8,5 @@ from .bfs import (  # noqa: F401
     MIXED_FRONTIERS, TWO_PHASE, VERTEX_FRONTIERS, BfsError, CsrGraph, CsrProblem,
     EnactorContractExpand, EnactorDirectionOptimizing, EnactorExpandContract, EnactorHybrid, EnactorTwoPhase,
     Stats,
-    barrier_benchmark, device_count, selftest_wave_primitives,
+    barrier_benchmark, device_count, read_coo, selftest_wave_primitives,
 )

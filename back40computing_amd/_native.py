@@ -23,6 +23,27 @@ class CsrStruct(ctypes.Structure):
     ]
 
 
+class CooStruct(ctypes.Structure):
+    _fields_ = [
+        ("nodes", ctypes.c_int32),
+        ("pairs", ctypes.c_int64),
+        ("src", c_int32_p),
+        ("dst", c_int32_p),
+    ]
+
+
+class DeviceCsrStruct(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int32),
+        ("nodes", ctypes.c_int32),
+        ("edges", ctypes.c_int64),
+        ("row_offsets", ctypes.c_void_p),
+        ("column_indices", ctypes.c_void_p),
+        ("symmetric", ctypes.c_int32),
+        ("build_ms", ctypes.c_double),
+    ]
+
+
 class BfsConfig(ctypes.Structure):
     _fields_ = [
         ("wave_gather_threshold", ctypes.c_int32),
@@ -138,6 +159,24 @@ def load():
          ctypes.c_int, ctypes.c_int, ctypes.c_int, csr_p)
     _sig(lib, "b40c_graph_sample_sources", ctypes.c_int, csr_p, i32, u64, c_int32_p)
     _sig(lib, "b40c_graph_free", None, csr_p)
+    coo_p = ctypes.POINTER(CooStruct)
+    _sig(lib, "b40c_read_dimacs", ctypes.c_int, ctypes.c_char_p, ctypes.c_int, coo_p)
+    _sig(lib, "b40c_read_metis", ctypes.c_int, ctypes.c_char_p, coo_p)
+    _sig(lib, "b40c_read_market", ctypes.c_int, ctypes.c_char_p, ctypes.c_int, coo_p)
+    _sig(lib, "b40c_coo_free", None, coo_p)
+    dcsr_p = ctypes.POINTER(DeviceCsrStruct)
+    _sig(lib, "b40c_device_csr_from_coo", ctypes.c_int, i32, i32, i64, c_int32_p, c_int32_p,
+         ctypes.c_int, ctypes.c_int, ctypes.c_int, dcsr_p)
+    _sig(lib, "b40c_device_csr_rmat", ctypes.c_int, i32, i32, i32, dbl, dbl, dbl, u64, ctypes.c_int, dcsr_p)
+    _sig(lib, "b40c_device_csr_transpose", ctypes.c_int, dcsr_p, dcsr_p)
+    _sig(lib, "b40c_device_csr_to_host", ctypes.c_int, dcsr_p, csr_p)
+    _sig(lib, "b40c_device_csr_free", None, dcsr_p)
+    _sig(lib, "b40c_scan_enactor_create", ctypes.c_int, i32, ctypes.POINTER(vp))
+    _sig(lib, "b40c_scan_enactor_destroy", None, vp)
+    _sig(lib, "b40c_scan", ctypes.c_int, vp, vp, vp, i64, i32, ctypes.POINTER(ctypes.c_uint32))
+    _sig(lib, "b40c_copy_if", ctypes.c_int, vp, vp, vp, i64, ctypes.c_int, ctypes.c_uint32,
+         ctypes.POINTER(i64), vp)
+    _sig(lib, "b40c_scan_last_ms", ctypes.c_int, vp, ctypes.POINTER(dbl))
 
     _sig(lib, "bfs_status_string", ctypes.c_char_p, ctypes.c_int)
     _sig(lib, "bfs_last_error_message", ctypes.c_char_p)
@@ -156,6 +195,8 @@ def load():
     _sig(lib, "bfs_problem_destroy", None, vp)
     _sig(lib, "bfs_problem_device_info", ctypes.c_int, vp, c_int32_p, c_int32_p,
          ctypes.POINTER(ctypes.c_int64))
+    _sig(lib, "bfs_problem_device_arrays", ctypes.c_int, vp, ctypes.POINTER(vp), ctypes.POINTER(vp),
+         ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp))
     _sig(lib, "bfs_get_level_log", ctypes.c_int, vp, i32, ctypes.POINTER(ctypes.c_uint32),
          ctypes.POINTER(dbl), c_int32_p)
     _sig(lib, "bfs_barrier_benchmark", ctypes.c_int, i32, i32, i32, ctypes.POINTER(dbl), c_int32_p)
@@ -163,6 +204,7 @@ def load():
     vpp = ctypes.POINTER(vp)
     _sig(lib, "bfs_pool_create", ctypes.c_int, i32, vpp)
     _sig(lib, "bfs_pool_add_graph", ctypes.c_int, vp, c_int32_p, c_int32_p, i32, i64, c_int32_p)
+    _sig(lib, "bfs_pool_add_device_graph", ctypes.c_int, vp, dcsr_p, c_int32_p)
     _sig(lib, "bfs_pool_graph_info", ctypes.c_int, vp, i32, c_int32_p, ctypes.POINTER(i64))
     _sig(lib, "bfs_pool_device_bytes", ctypes.c_int, vp, ctypes.POINTER(i64))
     _sig(lib, "bfs_pool_destroy", None, vp)

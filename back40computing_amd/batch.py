@@ -36,6 +36,15 @@ class GraphPool:
         self.edges.append(graph.edges)
         return gid.value
 
+    def add_device(self, dgraph):
+        """Adds a primitives.DeviceCsr (copied device to device)."""
+        gid = ctypes.c_int32()
+        _check(_native.load().bfs_pool_add_device_graph(self._h, ctypes.byref(dgraph._s), ctypes.byref(gid)),
+               "bfs_pool_add_device_graph")
+        self.nodes.append(dgraph.nodes)
+        self.edges.append(dgraph.edges)
+        return gid.value
+
     def device_bytes(self):
         b = ctypes.c_int64()
         _check(_native.load().bfs_pool_device_bytes(self._h, ctypes.byref(b)), "bfs_pool_device_bytes")

@@ -108,6 +108,13 @@ class CsrGraph:
         g._s.column_indices = _native.i32ptr(col) if len(col) else _native.i32ptr(np.zeros(1, np.int32))
         return g
 
+    @classmethod
+    def from_file(cls, kind, path, undirected=False):
+        """CSR of a DIMACS / METIS / Matrix Market file (read_coo), with
+        every pair kept (no dedup, no self-loop removal)."""
+        nodes, src, dst = read_coo(kind, path, undirected)
+        return cls.from_coo(nodes, src, dst)
+
     # -- access -----------------------------------------------------------
     @property
     def nodes(self):
@@ -148,6 +155,31 @@ class CsrGraph:
             self.free()
         except Exception:
             pass
+
+
+def read_coo(kind, path, undirected=False):
+    """(nodes, src, dst) of a graph file: kind is "dimacs", "metis" or
+    "market" (b40c/graph/builder/{dimacs,metis,market}.cuh grammars)."""
+    lib = _native.load()
+    c = _native.CooStruct()
+    p = path.encode() if isinstance(path, str) else path
+    if kind == "dimacs":
+        rc = lib.b40c_read_dimacs(p, int(undirected), ctypes.byref(c))
+    elif kind == "metis":
+        rc = lib.b40c_read_metis(p, ctypes.byref(c))
+    elif kind == "market":
+        rc = lib.b40c_read_market(p, int(undirected), ctypes.byref(c))
+    else:
+        raise ValueError("unknown graph file kind %r" % kind)
+    if rc != 0:
+        raise BfsError(1, "reading %s file %s failed with code %d" % (kind, path, rc))
+    try:
+        n = int(c.pairs)
+        src = np.ctypeslib.as_array(c.src, shape=(max(n, 1),))[:n].copy()
+        dst = np.ctypeslib.as_array(c.dst, shape=(max(n, 1),))[:n].copy()
+        return int(c.nodes), src, dst
+    finally:
+        lib.b40c_coo_free(ctypes.byref(c))
 
 
 class Stats:

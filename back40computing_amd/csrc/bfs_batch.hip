@@ -246,7 +246,7 @@ int batch_grid(const bfs_graph_pool *p, bool persistent, int blocks_per_cu)
     int occ = persistent ? batch_persistent_blocks_per_cu() : batch_level_blocks_per_cu();
     if (persistent) {
         // same co-residency margin as the single-source persistent kernel
-        int cap = std::max(1, std::min(4, occ / 2));
+        int cap = std::max(1, std::min(kMaxPersistentPerCu, occ / 2));
         return p->num_cus * std::min(cap, std::max(1, blocks_per_cu));
     }
     // level launches drain a shared tile counter, so the grid only sets how
@@ -335,6 +335,43 @@ bfs_status bfs_pool_add_graph(bfs_graph_pool *p, const int32_t *h_row, const int
                                            hipMemcpyHostToDevice) == hipSuccess);
         b40c_graph_free(&tr);
         if (!ok) return host_fail(BFS_ERR_OOM, "allocate or copy the transpose");
+        p->bytes += (int64_t) (rb + cb);
+    }
+    p->graphs.push_back(g);
+    bfs_status st = refresh_table(p);
+    if (st != BFS_OK) return st;
+    *graph_id = (int32_t) p->graphs.size() - 1;
+    return BFS_OK;
+}
+
+bfs_status bfs_pool_add_device_graph(bfs_graph_pool *p, const b40c_device_csr *dg, int32_t *graph_id)
+{
+    if (!p || !dg || !graph_id || !dg->row_offsets || dg->nodes <= 0 || dg->edges < 0 || dg->edges > INT32_MAX ||
+        (dg->edges > 0 && !dg->column_indices))
+        return host_fail(BFS_ERR_INVALID_ARG, "invalid device graph");
+    if (dg->device != p->device) return host_fail(BFS_ERR_INVALID_ARG, "device graph lives on another device");
+    std::lock_guard<std::mutex> lk(p->mu);
+    HIP_OK(hipSetDevice(p->device), "hipSetDevice");
+    PoolGraph g;
+    g.nodes = dg->nodes;
+    g.edges = dg->edges;
+    size_t rb = sizeof(int) * ((size_t) g.nodes + 1), cb = sizeof(int) * (size_t) std::max<int64_t>(g.edges, 1);
+    HIP_OK(hipMalloc(&g.row, rb), "allocate row offsets");
+    HIP_OK(hipMalloc(&g.col, cb), "allocate column indices");
+    HIP_OK(hipMemcpy(g.row, dg->row_offsets, rb, hipMemcpyDeviceToDevice), "copy row offsets");
+    if (g.edges > 0)
+        HIP_OK(hipMemcpy(g.col, dg->column_indices, sizeof(int) * (size_t) g.edges, hipMemcpyDeviceToDevice),
+               "copy columns");
+    p->bytes += (int64_t) (rb + cb);
+    if (dg->symmetric) {
+        g.rrow = g.row;
+        g.rcol = g.col;
+    } else {
+        // the in-neighbour CSR, sorted on the device as well
+        b40c_device_csr t{};
+        if (b40c_device_csr_transpose(dg, &t) != 0) return host_fail(BFS_ERR_OOM, "device transpose failed");
+        g.rrow = t.row_offsets;
+        g.rcol = t.column_indices;
         p->bytes += (int64_t) (rb + cb);
     }
     p->graphs.push_back(g);

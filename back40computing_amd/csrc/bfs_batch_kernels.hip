@@ -719,8 +719,8 @@ hipError_t launch_batch_persistent(const BatchArgs &a, int grid, hipStream_t st)
 {
     BatchArgs aa = a;
     void *args[] = {&aa};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&batch_persistent_kernel), dim3(grid),
-                                      dim3(kBlock), args, 0, st);
+    return launch_coresident(reinterpret_cast<const void *>(&batch_persistent_kernel), dim3(grid), dim3(kBlock), args,
+                             st);
 }
 
 hipError_t launch_batch_level(const BatchArgs &a, int grid, int *alive_out, hipStream_t st)

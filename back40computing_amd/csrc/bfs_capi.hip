@@ -105,21 +105,13 @@ struct bfs_problem {
 
 namespace {
 
-// Persistent grids stay well inside co-residency: at most half the reported
-// occupancy and at most kMaxPersistentPerCu workgroups per CU.  A launch at
-// full occupancy (8 per CU) went silent once and was never shown to be
+// Persistent grids follow kMaxPersistentPerCu (bfs_internal.hpp).  A launch
+// at full occupancy (8 per CU) went silent once and was never shown to be
 // safe, so that shape is not issued.
-constexpr int kMaxPersistentPerCu = 4;
-
 int persistent_grid(const bfs_problem *p, int loop, int max_grid_size)
 {
-    bool exact = p->cfg.cull_mode == BFS_CULL_EXACT;
-    int occ = persistent_max_blocks_per_cu(loop, p->mark_pred, exact);
-    int cap = std::max(1, std::min(kMaxPersistentPerCu, occ / 2));
-    int per_cu = std::min(cap, std::max(1, p->cfg.blocks_per_cu));
-    int grid = p->num_cus * per_cu;
-    if (max_grid_size > 0) grid = std::min(grid, max_grid_size);
-    return std::max(grid, 1);
+    return persistent_grid_for(p->num_cus, loop, p->mark_pred, p->cfg.cull_mode == BFS_CULL_EXACT,
+                               p->cfg.blocks_per_cu, max_grid_size);
 }
 
 int step_grid(const bfs_problem *p, int max_grid_size)
@@ -133,6 +125,7 @@ int step_grid(const bfs_problem *p, int max_grid_size)
 SearchArgs search_args(const bfs_problem *p, int grid)
 {
     SearchArgs s;
+    apply_search_config(s, p->cfg, grid);
     s.depth = p->d_depth;
     s.pred = p->d_pred;
     s.visited = p->d_visited;
@@ -145,29 +138,12 @@ SearchArgs search_args(const bfs_problem *p, int grid)
     s.hubs[0] = p->d_hubs;
     s.hubs[1] = p->d_hubs + kHubCapacity;
     s.hub_cap = kHubCapacity;
-    s.wave_t = p->cfg.wave_gather_threshold;
-    s.cta_t = p->cfg.cta_gather_threshold;
-    s.grid_t = p->cfg.grid_gather_threshold > 0 ? p->cfg.grid_gather_threshold : 0x7fffffff;
-    s.grab_max = p->cfg.tile_grab_max > 0 ? p->cfg.tile_grab_max : 1;
-    s.coherent_filter = p->cfg.claim_filter == 2;
-    s.spread = p->cfg.spread_steps == 1;
-    s.fast_t = p->cfg.fast_path_threshold;
-    s.sat_t = p->cfg.saturation_threshold > 0 ? p->cfg.saturation_threshold
-              : (p->cfg.saturation_threshold < 0 ? -1 : grid * 256);
-    s.instrument = p->cfg.instrument;
     size_t words = ((size_t) p->nodes + 31) / 32;
     s.fbm[0] = p->d_fbm;
     s.fbm[1] = p->d_fbm + words;
     s.marks = p->d_marks;
     s.nodes = p->nodes;
     s.total_edges = (unsigned long long) p->edges;
-    s.track_edges = 1;
-    s.do_alpha = p->cfg.do_alpha;
-    s.init_src = -1;
-    s.init_reset = 0;
-    s.init_label = 0;
-    s.host_head = nullptr;
-    s.do_beta = p->cfg.do_beta;
     return s;
 }
 
@@ -716,6 +692,19 @@ bfs_status bfs_problem_device_info(const bfs_problem *p, int32_t *num_cus, int32
     return BFS_OK;
 }
 
+bfs_status bfs_problem_device_arrays(const bfs_problem *p, const int32_t **d_row_offsets,
+                                    const int32_t **d_column_indices, const int32_t **d_in_row_offsets,
+                                    const int32_t **d_in_column_indices, int32_t **d_depths)
+{
+    if (!p) return fail(BFS_ERR_INVALID_ARG, "null problem");
+    if (d_row_offsets) *d_row_offsets = p->d_row;
+    if (d_column_indices) *d_column_indices = p->d_col;
+    if (d_in_row_offsets) *d_in_row_offsets = p->d_rrow;
+    if (d_in_column_indices) *d_in_column_indices = p->d_rcol;
+    if (d_depths) *d_depths = p->d_depth;
+    return BFS_OK;
+}
+
 bfs_status bfs_get_level_log(const bfs_problem *p, int32_t max_steps, uint32_t *lengths, double *start_us,
                              int32_t *steps)
 {
@@ -859,6 +848,37 @@ bfs_status bfs_selftest_wave_primitives(int32_t device, int32_t *mismatches)
 namespace b40c_hip {
 
 bfs_status host_fail(bfs_status s, const char *msg) { return fail(s, msg); }
+
+void apply_search_config(SearchArgs &s, const bfs_config &cfg, int grid)
+{
+    s.wave_t = cfg.wave_gather_threshold;
+    s.cta_t = cfg.cta_gather_threshold;
+    s.grid_t = cfg.grid_gather_threshold > 0 ? cfg.grid_gather_threshold : 0x7fffffff;
+    s.grab_max = cfg.tile_grab_max > 0 ? cfg.tile_grab_max : 1;
+    s.coherent_filter = cfg.claim_filter == 2;
+    s.spread = cfg.spread_steps == 1;
+    s.fast_t = cfg.fast_path_threshold;
+    s.sat_t = cfg.saturation_threshold > 0 ? cfg.saturation_threshold
+              : (cfg.saturation_threshold < 0 ? -1 : grid * 256);
+    s.instrument = cfg.instrument;
+    s.track_edges = 1;
+    s.do_alpha = cfg.do_alpha;
+    s.do_beta = cfg.do_beta;
+    s.init_src = -1;
+    s.init_reset = 0;
+    s.init_label = 0;
+    s.host_head = nullptr;
+}
+
+int persistent_grid_for(int num_cus, int loop, bool pred, bool exact, int blocks_per_cu, int max_grid_size)
+{
+    int occ = persistent_max_blocks_per_cu(loop, pred, exact);
+    int cap = std::max(1, std::min(kMaxPersistentPerCu, occ / 2));
+    int per_cu = std::min(cap, std::max(1, blocks_per_cu));
+    int grid = num_cus * per_cu;
+    if (max_grid_size > 0) grid = std::min(grid, max_grid_size);
+    return std::max(grid, 1);
+}
 
 bfs_status host_hip_fail(hipError_t e, const char *what) { return hip_fail(e, what); }
 

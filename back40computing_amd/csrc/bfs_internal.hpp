@@ -98,3 +98,31 @@ hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)
 hipError_t launch_wave_selftest(const int *in, int *out_scan, int *out_rank, int n, hipStream_t st);
 
 }  // namespace b40c_hip
+
+#include "b40c_bfs.h"
+
+namespace b40c_hip {
+// Persistent grids stay well inside co-residency: at most half the reported
+// occupancy and at most kMaxPersistentPerCu workgroups per CU (single-source,
+// batched and barrier-benchmark launches alike).
+constexpr int kMaxPersistentPerCu = 4;
+// Grid of a persistent launch on num_cus CUs under that rule.
+int persistent_grid_for(int num_cus, int loop, bool pred, bool exact, int blocks_per_cu, int max_grid_size);
+// Launches a persistent kernel, one whose workgroups meet at grid barriers
+// and so must all be resident at once.  A cooperative launch has the runtime
+// guarantee that.  The device has one cooperative queue, held by the first
+// process that makes a cooperative launch; in any other process the launch
+// reports hipErrorIllegalState, and the kernel then goes out as a plain
+// launch of the same grid -- which every caller sizes at most half the
+// kernel's occupancy (and at most kMaxPersistentPerCu per CU), so it is
+// resident at once unless other long-running work holds the CUs.  After the
+// first such refusal the process launches plainly.
+hipError_t launch_coresident(const void *kernel, dim3 grid, dim3 block, void **args, hipStream_t st);
+
+// Error helpers (they record the message bfs_last_error_message returns).
+bfs_status host_fail(bfs_status s, const char *msg);
+bfs_status host_hip_fail(hipError_t e, const char *what);
+// The tunables of a search (thresholds, switch points, grabs; no buffers)
+// from a config, with the search start left to the caller (init_src = -1).
+void apply_search_config(SearchArgs &s, const bfs_config &cfg, int grid);
+}  // namespace b40c_hip

@@ -23,6 +23,7 @@
 #include "bfs_internal.hpp"
 
 #include <algorithm>
+#include <atomic>
 
 namespace b40c_hip {
 
@@ -2024,6 +2025,18 @@ hipError_t launch_step(int phase, bool pred, bool exact, const GraphArgs &g, con
                  : launch_step_t<false, false>(phase, g, s, c, grid, st, launches);
 }
 
+hipError_t launch_coresident(const void *kernel, dim3 grid, dim3 block, void **args, hipStream_t st)
+{
+    static std::atomic<bool> plain{false};
+    if (!plain.load(std::memory_order_relaxed)) {
+        hipError_t e = hipLaunchCooperativeKernel(kernel, grid, block, args, 0, st);
+        if (e != hipErrorIllegalState) return e;
+        (void) hipGetLastError();  // the refusal is not an error of this launch
+        plain.store(true, std::memory_order_relaxed);
+    }
+    return hipLaunchKernel(kernel, grid, block, args, 0, st);
+}
+
 hipError_t launch_persistent(int loop, bool pred, bool exact, const GraphArgs &g, const SearchArgs &s, Cursor c,
                              int grid, hipStream_t st)
 {
@@ -2031,7 +2044,7 @@ hipError_t launch_persistent(int loop, bool pred, bool exact, const GraphArgs &g
     SearchArgs sa = s;
     Cursor ca = c;
     void *args[] = {&ga, &sa, &ca};
-    return hipLaunchCooperativeKernel(persistent_select(loop, pred, exact), dim3(grid), dim3(kBlock), args, 0, st);
+    return launch_coresident(persistent_select(loop, pred, exact), dim3(grid), dim3(kBlock), args, st);
 }
 
 int persistent_max_blocks_per_cu(int loop, bool pred, bool exact)
@@ -2066,8 +2079,8 @@ hipError_t launch_barrier_bench(Control *ctl, int iterations, int grid, hipStrea
     Control *c = ctl;
     int it = iterations;
     void *args[] = {&c, &it};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&barrier_bench_kernel), dim3(grid),
-                                      dim3(kBlock), args, 0, st);
+    return launch_coresident(reinterpret_cast<const void *>(&barrier_bench_kernel), dim3(grid), dim3(kBlock), args,
+                             st);
 }
 
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, hipStream_t st)

@@ -6,6 +6,7 @@
 // C++ with std::thread parallelism; results do not depend on the thread count
 // because every random draw is a pure function of (seed, item index).
 #include "b40c_graph.h"
+#include "graph_builder_internal.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -158,6 +159,20 @@ int csr_from_pairs(int32_t nodes, int64_t num_pairs, const int32_t *src, const i
 
 }  // namespace
 
+// Seeded vertex scramble of the R-MAT builders (host and device): a
+// Fisher-Yates shuffle of 0 .. 2^scale - 1 driven by one counter stream.
+void b40c_rmat_permutation(int32_t scale, uint64_t seed, std::vector<int32_t> *perm)
+{
+    const int32_t nodes = (int32_t) (1u << scale);
+    perm->resize(nodes);
+    for (int32_t v = 0; v < nodes; v++) (*perm)[v] = v;
+    Stream st(seed ^ 0x5CA3B1E5CA3B1E5Cull, 0);
+    for (int32_t v = nodes - 1; v > 0; v--) {
+        int32_t j = (int32_t) st.below((uint64_t) v + 1);
+        std::swap((*perm)[v], (*perm)[j]);
+    }
+}
+
 extern "C" {
 
 // Neighbour order (k-1, k+1, j-1, j+1) follows b40c/graph/builder/grid2d.cuh:78-100;
@@ -251,15 +266,7 @@ int b40c_graph_rmat(int32_t scale, int32_t edge_factor, double a, double b, doub
     const uint64_t t_abc = (uint64_t) ((a + b + c) * two32);
 
     std::vector<int32_t> perm;
-    if (scramble) {
-        perm.resize(nodes);
-        for (int32_t v = 0; v < nodes; v++) perm[v] = v;
-        Stream st(seed ^ 0x5CA3B1E5CA3B1E5Cull, 0);
-        for (int32_t v = nodes - 1; v > 0; v--) {
-            int32_t j = (int32_t) st.below((uint64_t) v + 1);
-            std::swap(perm[v], perm[j]);
-        }
-    }
+    if (scramble) b40c_rmat_permutation(scale, seed, &perm);
 
     int32_t *src = (int32_t *) std::malloc(sizeof(int32_t) * (size_t) samples);
     int32_t *dst = (int32_t *) std::malloc(sizeof(int32_t) * (size_t) samples);

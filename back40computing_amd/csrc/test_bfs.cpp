@@ -92,12 +92,73 @@ double cpu_bfs(const b40c_csr &g, int src, std::vector<int> &labels)
     return std::chrono::duration<double, std::milli>(t1 - t0).count();
 }
 
+struct SearchResult {
+    const char *name = nullptr;  // null: unknown strategy
+    bfs_status status = BFS_OK;
+    long long total_queued = 0;
+    int search_depth = 0;
+    double avg_duty = 0, ms = 0;
+};
+
+// One search with strategy s (test_bfs.cu:687-729): reset the problem for the
+// enactor's frontier type, enact, and collect the enactor's statistics.
+template <bool INSTRUMENT, typename Problem>
+SearchResult run_strategy(Problem &problem, int s, int src, bool iterative, int max_ctas, double qs, bool verbose)
+{
+    SearchResult r;
+    EnactorExpandContract<INSTRUMENT> ec(verbose);
+    EnactorContractExpand<INSTRUMENT> ce(verbose);
+    EnactorTwoPhase<INSTRUMENT> tp(verbose);
+    EnactorHybrid<INSTRUMENT> hy(verbose);
+    EnactorDirectionOptimizing<INSTRUMENT> dopt(verbose);
+    EnactorBase *base = nullptr;
+    switch (s) {
+    case 0:
+        r.name = "Expand-contract GPU BFS";
+        if ((r.status = problem.Reset(ec.GetFrontierType(), qs))) return r;
+        r.status = iterative ? ec.EnactIterativeSearch(problem, src, max_ctas) : ec.EnactFusedSearch(problem, src, max_ctas);
+        base = &ec;
+        break;
+    case 1:
+        r.name = "Contract-expand GPU BFS";
+        if ((r.status = problem.Reset(ce.GetFrontierType(), qs))) return r;
+        r.status = iterative ? ce.EnactIterativeSearch(problem, src, max_ctas) : ce.EnactFusedSearch(problem, src, max_ctas);
+        base = &ce;
+        break;
+    case 2:
+        r.name = "Two-phase GPU BFS";
+        if ((r.status = problem.Reset(tp.GetFrontierType(), qs))) return r;
+        r.status = iterative ? tp.EnactIterativeSearch(problem, src, max_ctas) : tp.EnactFusedSearch(problem, src, max_ctas);
+        base = &tp;
+        break;
+    case 3:
+        r.name = "Hybrid contract-expand + two-phase GPU BFS";
+        if ((r.status = problem.Reset(hy.GetFrontierType(), qs))) return r;
+        r.status = hy.EnactSearch(problem, src, max_ctas);
+        base = &hy;
+        break;
+    case 4:
+        r.name = "Direction-optimizing GPU BFS";
+        if ((r.status = problem.Reset(dopt.GetFrontierType(), qs))) return r;
+        r.status = iterative ? dopt.EnactIterativeSearch(problem, src, max_ctas) : dopt.EnactFusedSearch(problem, src, max_ctas);
+        base = &dopt;
+        break;
+    default: return r;
+    }
+    if (r.status == BFS_OK) {
+        base->GetStatistics(r.total_queued, r.search_depth, r.avg_duty);
+        r.ms = base->ElapsedMillis();
+    }
+    return r;
+}
+
 void usage()
 {
     std::printf(
         "\ntest_bfs <graph type> <graph type args> [--device=<i>] [--i=<iterations>]\n"
         "         [--src=<id|randomize>] [--queue-sizing=<f>] [--mark-pred]\n"
-        "         [--strategy=<s>[,<s>]*] [--iterative] [--max-ctas=<n>] [--quick] [--v]\n\n"
+        "         [--strategy=<s>[,<s>]*] [--iterative] [--max-ctas=<n>] [--instrumented]\n"
+        "         [--quick] [--v]\n\n"
         "Graph types: grid2d <width> | grid3d <width> | rmat <scale> <edge-factor> |\n"
         "             er <n> <m> | random <n> <m> [--undirected]\n"
         "Strategies: 0 expand-contract, 1 contract-expand, 2 two-phase, 3 hybrid (default 3),\n"
@@ -137,6 +198,7 @@ int main(int argc, char **argv)
     const int max_ctas = std::atoi(args.get("max-ctas", "0").c_str());
     const bool quick = args.flag("quick"), verbose = args.flag("v"), mark_pred = args.flag("mark-pred");
     const bool iterative = args.flag("iterative");
+    const bool instrumented = args.flag("instrumented");
     std::vector<int> strategies;
     {
         std::string s = args.get("strategy", "3");
@@ -179,54 +241,18 @@ int main(int argc, char **argv)
             if (it > 0) cpu_rate.Update(ev / (cpu_ms * 1000.0));
             std::printf("CPU BFS from %d: %.3f ms, %.3f MiEdges/s\n", src, cpu_ms, ev / (cpu_ms * 1000.0));
             for (int s : strategies) {
-                EnactorExpandContract<false> ec(verbose);
-                EnactorContractExpand<false> ce(verbose);
-                EnactorTwoPhase<false> tp(verbose);
-                EnactorHybrid<false> hy(verbose);
-                EnactorDirectionOptimizing<false> dopt(verbose);
-                EnactorBase *base = nullptr;
-                bfs_status st = BFS_OK;
-                const char *name = "";
-                switch (s) {
-                case 0:
-                    name = "Expand-contract GPU BFS";
-                    if ((st = problem.Reset(ec.GetFrontierType(), qs))) break;
-                    st = iterative ? ec.EnactIterativeSearch(problem, src, max_ctas) : ec.EnactFusedSearch(problem, src, max_ctas);
-                    base = &ec;
-                    break;
-                case 1:
-                    name = "Contract-expand GPU BFS";
-                    if ((st = problem.Reset(ce.GetFrontierType(), qs))) break;
-                    st = iterative ? ce.EnactIterativeSearch(problem, src, max_ctas) : ce.EnactFusedSearch(problem, src, max_ctas);
-                    base = &ce;
-                    break;
-                case 2:
-                    name = "Two-phase GPU BFS";
-                    if ((st = problem.Reset(tp.GetFrontierType(), qs))) break;
-                    st = iterative ? tp.EnactIterativeSearch(problem, src, max_ctas) : tp.EnactFusedSearch(problem, src, max_ctas);
-                    base = &tp;
-                    break;
-                case 3:
-                    name = "Hybrid contract-expand + two-phase GPU BFS";
-                    if ((st = problem.Reset(hy.GetFrontierType(), qs))) break;
-                    st = hy.EnactSearch(problem, src, max_ctas);
-                    base = &hy;
-                    break;
-                case 4:
-                    name = "Direction-optimizing GPU BFS";
-                    if ((st = problem.Reset(dopt.GetFrontierType(), qs))) break;
-                    st = iterative ? dopt.EnactIterativeSearch(problem, src, max_ctas)
-                                   : dopt.EnactFusedSearch(problem, src, max_ctas);
-                    base = &dopt;
-                    break;
-                default: std::fprintf(stderr, "unknown strategy %d\n", s); return 1;
+                SearchResult r = instrumented
+                                     ? run_strategy<true>(problem, s, src, iterative, max_ctas, qs, verbose)
+                                     : run_strategy<false>(problem, s, src, iterative, max_ctas, qs, verbose);
+                if (!r.name) {
+                    std::fprintf(stderr, "unknown strategy %d\n", s);
+                    return 1;
                 }
-                if (st != BFS_OK) return 1;
-                long long queued;
-                int depth;
-                double duty;
-                base->GetStatistics(queued, depth, duty);
-                double ms = base->ElapsedMillis();
+                if (r.status != BFS_OK) return 1;
+                const char *name = r.name;
+                const long long queued = r.total_queued;
+                const int depth = r.search_depth;
+                const double duty = r.avg_duty, ms = r.ms;
                 const char *validity = "skipped";
                 if (!quick) {
                     if (problem.ExtractResults(labels.data())) return 1;
@@ -255,10 +281,18 @@ int main(int argc, char **argv)
                 } else {
                     double rate = ev / (ms * 1000.0);
                     double sd = std::sqrt(rates[s].Update(rate));
+                    // duplicates put through the queues, as a share of what an exact
+                    // search queues: the traversed edges for edge frontiers (test_bfs.cu:
+                    // 369-373), the visited vertices for vertex frontiers
+                    const bool vertex_frontier = s == 0 || s == 4;
+                    const double unit = vertex_frontier ? (double) nv : (double) ev;
+                    const double redundant = queued > 0 ? 100.0 * ((double) queued - unit) / unit : 0.0;
                     std::printf("[%s] finished. Validity: %s\n  elapsed: %.3f ms, rate: %.3f MiEdges/s, "
-                                "search_depth: %d\n  src: %d, nodes visited: %lld, edges visited: %lld, total "
-                                "queued: %lld\n  [Rate MiEdges/s]: u: %.3f, s: %.3f\n",
-                                name, validity, ms, rate, depth, src, nv, ev, queued, rates[s].mean, sd);
+                                "search_depth: %d\n", name, validity, ms, rate, depth);
+                    if (duty != 0) std::printf("  avg cta duty: %.2f%%\n", duty * 100);
+                    std::printf("  src: %d, nodes visited: %lld, edges visited: %lld, total queued: %lld, "
+                                "redundant work: %.2f%%\n  [Rate MiEdges/s]: u: %.3f, s: %.3f\n",
+                                src, nv, ev, queued, redundant, rates[s].mean, sd);
                 }
             }
         }

@@ -8,24 +8,31 @@ test/bfs/test_bfs.cu:360-367) / wall time of the K timed steps.  The default
 strategy is direction-optimizing; the top-down expand-contract rate over the
 same sources is reported beside it.
 
-The roofline block is for the persistent traversal kernel: algorithmic
-bytes per launch, B_alg = 8 * E_vis + 20 * V_vis (the top-down per-edge and
-per-vertex bytes), over the average launch time measured with HIP events on
-the library's stream (the enact's elapsed_ms).  HBM traffic per launch comes
-from the committed counter summary under profiles/ when present.
+roofline: the persistent traversal kernel's algorithmic bytes per launch,
+B_alg = 8 * E_vis + 20 * V_vis (the top-down per-edge and per-vertex bytes),
+over the average launch time measured with HIP events on the library's
+stream (the enact's elapsed_ms).  HBM traffic per launch, counter GB/s and
+bytes per traversed edge come from the committed counter summaries under
+profiles/ (rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this script), labelled
+with the commit they were measured at.
+
+lattices: the 2D 5-point lattice 1000 x 1000 and the 3D 7-point lattice
+300^3 from vertex 0 (1999 and 898 levels), every strategy, median of 5
+searches each, beside the serial CPU BFS on the same graph.
 
 batched: 256 queries over 64 R-MAT graphs (scale 14..20, edge factor
 8..32) advanced together per level in one persistent launch, against the
 same queries run back to back as single-source searches; queries/s and
-per-query completion-time percentiles.
+per-query completion-time percentiles, by wall clock and by kernel time.
 
-serving (--serving): Poisson arrivals against resident Erdos-Renyi graphs
-(n = 2^16..2^23, average degree 8) and one R-MAT graph (scale 24, edge factor
-16 by default), at several rates; p50/p99 latency and queries/s.
+serving: Poisson arrivals against resident Erdos-Renyi graphs (n = 2^16 ..
+2^23, average degree 8) and one R-MAT graph (scale 24, edge factor 16), at
+a few stated rates; p50/p99 latency and queries/s.
 
 cpu_baseline times the serial reference BFS restated in oracle/ (one
-thread, traversal loop only) on a sample of the same sources, and checks the
-GPU labels of those sources against it.
+thread pinned to one core, traversal loop only, median of 5 runs per
+source) on a sample of the same sources, and checks the GPU labels of those
+sources against it.
 
 Prints exactly one JSON line on stdout; progress goes to stderr.
 """
@@ -42,6 +49,7 @@ sys.path.insert(0, ROOT)
 METRIC = "BFS GTEPS (R-MAT s22 ef16) + batched queries/s & p99 vs single-source"
 HBM_PEAK_GBS = 8000.0
 STRATEGIES = {"ec": 0, "ce": 1, "tp": 2, "hybrid": 3, "do": 4}
+CPU_RUNS = 5
 
 
 def log(*a):
@@ -59,6 +67,36 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
+def host_info():
+    aff = sorted(os.sched_getaffinity(0))
+    return {"cpu": cpu_model(), "nproc": os.cpu_count(), "usable_cores": len(aff)}
+
+
+class PinnedCore:
+    """Pins the calling thread to one core of its allowed set (taskset -c
+    <core> for the serial baseline), restoring the set on exit."""
+
+    def __enter__(self):
+        self.old = os.sched_getaffinity(0)
+        self.core = min(self.old)
+        os.sched_setaffinity(0, {self.core})
+        return self
+
+    def __exit__(self, *exc):
+        os.sched_setaffinity(0, self.old)
+
+
+def cpu_bfs_ms(oracle, row, col, src, runs=CPU_RUNS):
+    """Median traversal-loop time of `runs` serial searches (pinned), and the labels."""
+    import numpy as np
+    times, labels = [], None
+    with PinnedCore():
+        for _ in range(runs):
+            labels, _, ms = oracle.bfs(row, col, int(src))
+            times.append(ms)
+    return float(np.median(times)), labels
+
+
 def load_traffic(workload):
     path = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
     if not os.path.exists(path):
@@ -70,6 +108,57 @@ def load_traffic(workload):
 def pct(a, q):
     import numpy as np
     return round(float(np.percentile(np.asarray(a), q)), 4) if len(a) else None
+
+
+def bench_lattices(b40c, np, oracle, tuned_single, cpu_runs):
+    """Configurations 1 and 3: every strategy from vertex 0, median of 5."""
+    out = {}
+    for name, builder, width in [("grid2d_1000", b40c.CsrGraph.grid2d, 1000),
+                                 ("grid3d_300", b40c.CsrGraph.grid3d, 300)]:
+        t = time.time()
+        g = builder(width)
+        deg = g.degrees()
+        p = b40c.CsrProblem().from_host_problem(g)
+        if tuned_single:
+            p.set_config(**tuned_single)
+        res = {"vertices": g.nodes, "edges": g.edges, "source": 0, "strategies": {}}
+        e_vis = v_vis = None
+        for sname, strategy, fused in [("ec", 0, True), ("ce", 1, True), ("tp", 2, True), ("tp_iter", 2, False),
+                                       ("hybrid", 3, True), ("do", 4, True)]:
+            ft = b40c.VERTEX_FRONTIERS if strategy in (0, 4) else b40c.EDGE_FRONTIERS
+            p.reset(ft, 1.3)
+            st = p.enact(strategy, 0, fused=fused)  # warm-up
+            if e_vis is None:
+                d = p.extract_depths()
+                e_vis, v_vis = int(deg[d >= 0].sum()), int((d >= 0).sum())
+                res["levels"] = st.search_depth
+            ms = [p.enact(strategy, 0, fused=fused).elapsed_ms for _ in range(5)]
+            med = float(np.median(ms))
+            res["strategies"][sname] = {"ms": round(med, 3), "gteps": round(e_vis / (med * 1e-3) / 1e9, 3),
+                                        "launches": st.kernel_launches}
+        best = min(res["strategies"], key=lambda k: res["strategies"][k]["ms"])
+        bms = res["strategies"][best]["ms"]
+        b_alg = 8.0 * e_vis + 20.0 * v_vis
+        res["best"] = best
+        res["roofline"] = {"bound": "hbm", "algorithmic_bytes": b_alg,
+                           "achieved": round(b_alg / (bms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(b_alg / (bms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                           "us_per_level": round(1e3 * bms / res["levels"], 3)}
+        if cpu_runs > 0:
+            cms, labels = cpu_bfs_ms(oracle, g.row_offsets, g.column_indices, 0, cpu_runs)
+            res["cpu_baseline"] = {"ms": round(cms, 3), "gteps": round(e_vis / (cms * 1e-3) / 1e9, 4), "cores": 1,
+                                   "runs": cpu_runs, "kind": "port", "pinned": True}
+            res["gpu_over_cpu"] = round(cms / bms, 2)
+            p.reset(b40c.VERTEX_FRONTIERS, 1.3)
+            p.enact(0, 0)
+            res["gpu_labels_match"] = oracle.compare(p.extract_depths(), labels) == -1
+        p.destroy()
+        del g
+        out[name] = res
+        log("%s: best %s %.3f ms (%.1f us/level)%s (%.1f s)" % (
+            name, best, bms, res["roofline"]["us_per_level"],
+            ", cpu %.3f ms" % res["cpu_baseline"]["ms"] if "cpu_baseline" in res else "", time.time() - t))
+    return out
 
 
 def bench_batched(b40c, np, reps, tuned_batch, tuned_single):
@@ -91,15 +180,22 @@ def bench_batched(b40c, np, reps, tuned_batch, tuned_single):
         b.enact(gids, srcs, direction_optimizing=True, mode=mode)  # warm-up
         best = None
         for _ in range(reps):
+            t0 = time.perf_counter()
             st = b.enact(gids, srcs, direction_optimizing=True, mode=mode)
+            call_ms = (time.perf_counter() - t0) * 1e3
             if best is None or st["elapsed_ms"] < best["elapsed_ms"]:
                 best = st
+                best_call = call_ms
                 lat = b.latencies_ms()
         out[name] = {"ms": round(best["elapsed_ms"], 4),
                      "queries_per_s": round(len(gids) / (best["elapsed_ms"] * 1e-3), 1),
+                     # the whole call: query seeding, the level loop, the
+                     # statistics read-back
+                     "call_ms": round(best_call, 4),
+                     "call_queries_per_s": round(len(gids) / (best_call * 1e-3), 1),
                      "latency_p50_ms": pct(lat, 50), "latency_p99_ms": pct(lat, 99),
                      "levels": best["levels"], "launches": best["launches"]}
-        log("batched %s: %.3f ms for %d queries" % (name, best["elapsed_ms"], len(gids)))
+        log("batched %s: %.3f ms for %d queries (call %.3f ms)" % (name, best["elapsed_ms"], len(gids), best_call))
     b.destroy()
 
     # Back-to-back single-source searches of the same queries (one resident
@@ -126,29 +222,39 @@ def bench_batched(b40c, np, reps, tuned_batch, tuned_single):
     pool.destroy()
     out["single_source_back_to_back"] = {
         "ms": round(wall * 1e3, 3), "queries_per_s": round(len(gids) / wall, 1),
+        "kernel_ms_sum": round(float(kms.sum()), 3),
         "latency_p50_ms": pct(done, 50), "latency_p99_ms": pct(done, 99),
         "per_query_kernel_ms_p50": pct(kms, 50), "per_query_kernel_ms_p99": pct(kms, 99)}
-    out["speedup_queries_per_s"] = round(out["persistent"]["queries_per_s"] /
-                                         out["single_source_back_to_back"]["queries_per_s"], 2)
-    log("batched speedup over back-to-back: %.2fx" % out["speedup_queries_per_s"])
+    pers = out["persistent"]
+    # wall-clock ratio (each side's host overheads included) and the
+    # kernel-only ratio (batch level loop vs the sum of single-search kernels)
+    out["speedup_queries_per_s"] = round(pers["queries_per_s"] / out["single_source_back_to_back"]["queries_per_s"], 2)
+    out["speedup_call_vs_wall"] = round(wall * 1e3 / pers["call_ms"], 2)
+    out["speedup_kernel_only"] = round(float(kms.sum()) / pers["ms"], 2)
+    log("batched speedup over back-to-back: %.2fx wall, %.2fx call-inclusive, %.2fx kernel-only" % (
+        out["speedup_queries_per_s"], out["speedup_call_vs_wall"], out["speedup_kernel_only"]))
     return out, graphs, gids, srcs
 
 
-def bench_serving(b40c, np, rmat_scale, rates, queries_per_rate):
+def bench_serving(b40c, np, oracle, rmat_scale, rates, queries_per_rate, cpu_runs):
     from back40computing_amd.batch import GraphPool, Server
     from back40computing_amd.workloads import poisson_arrivals, serving_pool
 
     t = time.time()
     names, graphs = serving_pool(rmat_scale=rmat_scale)
-    log("serving pool: %s (%.1f s)" % (", ".join(names), time.time() - t))
+    build_s = time.time() - t
+    log("serving pool: %s (%.1f s)" % (", ".join(names), build_s))
+    t = time.time()
     pool = GraphPool()
     for g in graphs:
         pool.add(g)
+    upload_s = time.time() - t
     rng = np.random.default_rng(0x5E12)
     src_tables = [g.sample_sources(64, seed=7 + i) for i, g in enumerate(graphs)]
     budget = 16 << 30
+    max_slots, levels_per_tick = 128, 2
     results = []
-    srv = Server(pool, max_slots=128, levels_per_tick=2, hbm_budget_bytes=budget)
+    srv = Server(pool, max_slots=max_slots, levels_per_tick=levels_per_tick, hbm_budget_bytes=budget)
     # page-locked result buffers, allocated and touched before timing
     warm = [[srv._buffer(gi) for _ in range(12)] for gi in range(len(graphs))]
     for gi, bufs in enumerate(warm):
@@ -157,9 +263,11 @@ def bench_serving(b40c, np, rmat_scale, rates, queries_per_rate):
             srv.recycle(gi, b)
     import queue
     import threading
+    all_picks = []
     for rate in rates:
         arrivals = poisson_arrivals(rate, queries_per_rate, seed=int(rate))
         picks = [(int(rng.integers(0, len(graphs))), int(rng.integers(0, 64))) for _ in range(queries_per_rate)]
+        all_picks += picks
         before = srv.stats()
         lat, finish, submits = [], [], []
         handed = queue.Queue()
@@ -200,7 +308,25 @@ def bench_serving(b40c, np, rmat_scale, rates, queries_per_rate):
             rate, results[-1]["queries_per_s"], results[-1]["latency_p50_ms"], results[-1]["latency_p99_ms"]))
     srv.destroy()
     pool.destroy()
-    return {"graphs": names, "hbm_budget_bytes": budget, "max_slots": 128, "levels_per_tick": 2, "rates": results}
+    out = {"graphs": names, "vertices": [g.nodes for g in graphs], "edges": [g.edges for g in graphs],
+           "host_build_s": round(build_s, 2), "upload_s": round(upload_s, 2),
+           "hbm_budget_bytes": budget, "max_slots": max_slots, "levels_per_tick": levels_per_tick,
+           "rates": results}
+    if cpu_runs > 0:
+        # serial time of one query per graph (the first pick on it), weighted
+        # by how often the offered mix picks each graph
+        per_graph = []
+        for gi, g in enumerate(graphs):
+            ms, _ = cpu_bfs_ms(oracle, g.row_offsets, g.column_indices, int(src_tables[gi][0]),
+                               1 if g.edges > 1e8 else cpu_runs)
+            per_graph.append(ms)
+        freq = np.bincount([gi for gi, _ in all_picks], minlength=len(graphs)) / max(1, len(all_picks))
+        mean_ms = float(np.dot(freq, per_graph))
+        out["cpu_baseline"] = {"queries_per_s": round(1e3 / mean_ms, 2), "per_graph_ms": [round(x, 2) for x in per_graph],
+                               "cores": 1, "kind": "port", "pinned": True,
+                               "sample": "one source per graph, median of %d runs (1 run for graphs over 1e8 edges), "
+                                         "weighted by the offered mix" % cpu_runs}
+    return out
 
 
 def main():
@@ -213,16 +339,18 @@ def main():
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--sources", type=int, default=64)
-    ap.add_argument("--cpu-sample", type=int, default=12, help="sources timed on the CPU oracle (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=4, help="sources timed on the CPU oracle (0: skip all CPU legs)")
+    ap.add_argument("--cpu-runs", type=int, default=CPU_RUNS, help="serial runs per CPU sample (median)")
     ap.add_argument("--config", default="", help="comma list of bfs_config overrides, e.g. blocks_per_cu=2")
     ap.add_argument("--no-tuned", action="store_true",
                     help="library defaults instead of the autotuned table (back40computing_amd/tuned.json)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched section")
     ap.add_argument("--batched-reps", type=int, default=5)
-    ap.add_argument("--serving", action="store_true", help="add the serving sweep")
+    ap.add_argument("--no-lattices", action="store_true", help="skip the lattice section")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving sweep")
     ap.add_argument("--serving-rmat-scale", type=int, default=24)
-    ap.add_argument("--serving-rates", default="100,200,400,800,1600,3200")
-    ap.add_argument("--serving-queries", type=int, default=1000)
+    ap.add_argument("--serving-rates", default="200,1000")
+    ap.add_argument("--serving-queries", type=int, default=200)
     ap.add_argument("--calibrate-bytes", type=int, default=0,
                     help="also run the plain copy kernel over this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -239,6 +367,9 @@ def main():
     if b40c.device_count() < 1:
         raise SystemExit("no HIP device visible")
     lib = b40c.bfs._native.load()
+    oracle = None
+    if args.cpu_sample > 0:
+        import oracle
 
     t = time.time()
     g = b40c.CsrGraph.rmat(args.scale, args.edge_factor)
@@ -301,35 +432,51 @@ def main():
     achieved = float(b_alg.mean() / (kernel_ms.mean() * 1e-3) / 1e9)
     workload = "rmat%d_ef%d_%dsrc" % (args.scale, args.edge_factor, len(srcs))
     traffic = load_traffic(workload)
-    traffic_bytes = None
+    traffic_bytes, measured = None, None
     if traffic and fused and args.strategy in traffic.get("strategies", {}):
-        traffic_bytes = traffic["strategies"][args.strategy]["hbm_bytes_per_launch"]
+        tr = traffic["strategies"][args.strategy]
+        traffic_bytes = tr["hbm_bytes_per_launch"]
+        measured = {"profiled_at": traffic.get("commit"),
+                    "hbm_bytes_per_launch": traffic_bytes,
+                    "bytes_per_traversed_edge": round(traffic_bytes / float(e_vis.mean()), 3),
+                    # counter bytes over this run's average launch time
+                    "counter_gbs": round(traffic_bytes / (kernel_ms.mean() * 1e-3) / 1e9, 2),
+                    "counter_frac": round(traffic_bytes / (kernel_ms.mean() * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        for k in ("kernel_us", "waves_per_cu", "grbm_gui_active"):
+            if k in tr:
+                measured[k] = tr[k]
+        if "phases" in traffic:
+            measured["phases"] = traffic["phases"]
 
     # Top-down expand-contract over the same sources, for reference.
     top_down = None
     if strategy != b40c.EXPAND_CONTRACT:
         p.reset(b40c.VERTEX_FRONTIERS, 1.3)
-        tms = [p.enact(b40c.EXPAND_CONTRACT, int(s)).elapsed_ms for s in srcs[:16]]
-        top_down = {"strategy": "ec", "fused": True, "sources": 16,
-                    "gteps_kernel_mean": round(float(np.mean(e_vis[:16] / (np.array(tms) * 1e-3) / 1e9)), 3),
+        tms = [p.enact(b40c.EXPAND_CONTRACT, int(s)).elapsed_ms for s in srcs]
+        top_down = {"strategy": "ec", "fused": True, "sources": len(srcs),
+                    "gteps_kernel_mean": round(float(np.mean(e_vis / (np.array(tms) * 1e-3) / 1e9)), 3),
                     "kernel_ms_mean": round(float(np.mean(tms)), 4)}
+        if traffic and "ec" in traffic.get("strategies", {}):
+            tb = traffic["strategies"]["ec"]["hbm_bytes_per_launch"]
+            top_down["measured"] = {"profiled_at": traffic.get("commit"), "hbm_bytes_per_launch": tb,
+                                    "bytes_per_traversed_edge": round(tb / float(e_vis.mean()), 3),
+                                    "counter_gbs": round(tb / (np.mean(tms) * 1e-3) / 1e9, 2)}
 
     cpu = None
     if args.cpu_sample > 0:
-        import oracle
         row, col = g.row_offsets, g.column_indices
         cpu_edges, cpu_ms, parity = 0, 0.0, True
         n = min(args.cpu_sample, len(srcs))
         for i in range(n):
-            labels, _, ms = oracle.bfs(row, col, int(srcs[i]))
+            ms, labels = cpu_bfs_ms(oracle, row, col, int(srcs[i]), args.cpu_runs)
             cpu_ms += ms
             cpu_edges += int(e_vis[i])
             if oracle.compare(kept[i], labels) != -1:
                 parity = False
         cpu = {"value": round(cpu_edges / (cpu_ms * 1e-3) / 1e9, 4), "unit": "GTEPS", "cores": 1, "kind": "port",
                "sample": "%d of the %d sources, serial FIFO BFS (oracle/bfs_oracle.c), traversal loop wall time, "
-                         "host %s" % (n, len(srcs), cpu_model()),
-               "ms_per_source": round(cpu_ms / n, 2), "gpu_labels_match": parity}
+                         "one thread pinned to one core, median of %d runs per source" % (n, len(srcs), args.cpu_runs),
+               "host": host_info(), "ms_per_source": round(cpu_ms / n, 2), "gpu_labels_match": parity}
         log("cpu baseline: %.3f GTEPS over %d sources, GPU parity %s" % (cpu["value"], n, parity))
 
     p.destroy()
@@ -340,21 +487,26 @@ def main():
         rc = lib.bfs_copy_calibration(0, args.calibrate_bytes, ctypes.byref(ms))
         log("calibration copy of %d bytes: rc %d, %.3f ms" % (args.calibrate_bytes, rc, ms.value))
 
+    lattices = None
+    if not args.no_lattices:
+        lattices = bench_lattices(b40c, np, oracle, tuned_single, args.cpu_runs if args.cpu_sample > 0 else 0)
+
     batched = None
     if not args.no_batched:
         batched, bgraphs, bgids, bsrcs = bench_batched(b40c, np, args.batched_reps, tuned_batch, tuned_single)
         if cpu is not None:
-            import oracle
             sample = range(0, len(bgids), 8)
-            ms = sum(oracle.bfs(bgraphs[bgids[q]].row_offsets, bgraphs[bgids[q]].column_indices,
-                                int(bsrcs[q]))[2] for q in sample)
+            with PinnedCore():
+                ms = sum(oracle.bfs(bgraphs[bgids[q]].row_offsets, bgraphs[bgids[q]].column_indices,
+                                    int(bsrcs[q]))[2] for q in sample)
             cpu["batched_queries_per_s"] = round(len(sample) / (ms * 1e-3), 1)
-            cpu["batched_sample"] = "%d of the 256 batched queries, serial oracle, back to back" % len(sample)
+            cpu["batched_sample"] = "%d of the 256 batched queries, serial oracle, back to back, pinned" % len(sample)
         del bgraphs
     serving = None
-    if args.serving:
-        serving = bench_serving(b40c, np, args.serving_rmat_scale,
-                                [float(r) for r in args.serving_rates.split(",")], args.serving_queries)
+    if not args.no_serving:
+        serving = bench_serving(b40c, np, oracle, args.serving_rmat_scale,
+                                [float(r) for r in args.serving_rates.split(",")], args.serving_queries,
+                                args.cpu_runs if args.cpu_sample > 0 else 0)
 
     out = {
         "metric": METRIC,
@@ -372,15 +524,18 @@ def main():
         "config": {"workload": workload, "graph": "R-MAT a=0.57 b=0.19 c=0.19 d=0.05, symmetrised, "
                    "self-loops and duplicates removed, scrambled ids", "scale": args.scale,
                    "edge_factor": args.edge_factor, "vertices": int(len(deg)), "directed_edges": int(deg.sum()),
-                   "sources": len(srcs), "strategy": args.strategy, "fused": fused, "overrides": overrides,
-                   "tuned": bool(tuned_single), "tuned_single": tuned_single, "tuned_batch": tuned_batch},
+                   "sources": len(srcs), "step": "one search from each of the %d sources" % len(srcs),
+                   "searches_timed": args.steps * len(srcs), "strategy": args.strategy, "fused": fused,
+                   "overrides": overrides, "tuned": bool(tuned_single), "tuned_single": tuned_single,
+                   "tuned_batch": tuned_batch},
         "gteps_kernel_mean": round(float(per_src.mean()), 3),
         "gteps_kernel_hmean": round(float(len(per_src) / np.sum(1.0 / per_src)), 3),
         "kernel_ms_mean": round(float(kernel_ms.mean()), 4),
         "top_down": top_down,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_bytes,
-                     "algorithmic_bytes_per_launch": float(b_alg.mean())},
+                     "algorithmic_bytes_per_launch": float(b_alg.mean()), "measured": measured},
+        "lattices": lattices,
         "batched": batched,
         "serving": serving,
         "cpu_baseline": cpu,

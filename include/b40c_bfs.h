@@ -170,6 +170,13 @@ void bfs_problem_destroy(bfs_problem *p);
 bfs_status bfs_problem_device_info(const bfs_problem *p, int32_t *num_cus, int32_t *grid_size,
                                    int64_t *device_bytes);
 
+/* addition: device addresses of a problem's CSR (as uploaded), its
+ * in-neighbour CSR (the same arrays when the graph is symmetric) and its
+ * depth labels.  Any pointer argument may be NULL. */
+bfs_status bfs_problem_device_arrays(const bfs_problem *p, const int32_t **d_row_offsets,
+                                    const int32_t **d_column_indices, const int32_t **d_in_row_offsets,
+                                    const int32_t **d_in_column_indices, int32_t **d_depths);
+
 /* addition: per-step record of the last enact (first 64 steps): input
  * frontier length and start time in microseconds relative to step 0.
  * Returns the number of steps recorded in *steps. */

@@ -83,13 +83,23 @@ class EnactorBase {
   protected:
     bfs_strategy strategy_;
     bool debug_;
+    bool instrument_;
     bfs_stats stats_{};
-    explicit EnactorBase(bfs_strategy s, bool debug) : strategy_(s), debug_(debug) {}
+    EnactorBase(bfs_strategy s, bool debug, bool instrument) : strategy_(s), debug_(debug), instrument_(instrument) {}
 
+    // INSTRUMENT enactors (e.g. enactor_contract_expand.cuh:60-61) turn on the
+    // problem's duty statistics (bfs_config::instrument) for their searches.
     template <typename Problem>
     bfs_status Enact(Problem &p, int src, int fused, int max_grid_size)
     {
-        bfs_status s = bfs_enact(p.handle, strategy_, fused, src, max_grid_size, &stats_);
+        bfs_config cfg;
+        bfs_status s = bfs_problem_get_config(p.handle, &cfg);
+        if (s == BFS_OK && cfg.instrument != (instrument_ ? 1 : 0)) {
+            cfg.instrument = instrument_ ? 1 : 0;
+            s = bfs_problem_set_config(p.handle, &cfg);
+        }
+        if (s != BFS_OK) return Perror(s, "Enactor configuration failed", __FILE__, __LINE__);
+        s = bfs_enact(p.handle, strategy_, fused, src, max_grid_size, &stats_);
         if (debug_ && s == BFS_OK)
             std::printf("DEBUG: %d launches, %.3f ms\n", stats_.kernel_launches, stats_.elapsed_ms);
         return Perror(s, "Enactor search failed", __FILE__, __LINE__);
@@ -113,7 +123,7 @@ class EnactorBase {
 template <bool INSTRUMENT>
 class EnactorExpandContract : public EnactorBase {
   public:
-    explicit EnactorExpandContract(bool debug = false) : EnactorBase(BFS_EXPAND_CONTRACT, debug) {}
+    explicit EnactorExpandContract(bool debug = false) : EnactorBase(BFS_EXPAND_CONTRACT, debug, INSTRUMENT) {}
     template <typename Problem>
     bfs_status EnactFusedSearch(Problem &p, int src, int max_grid_size = 0)
     {
@@ -129,7 +139,7 @@ class EnactorExpandContract : public EnactorBase {
 template <bool INSTRUMENT>
 class EnactorContractExpand : public EnactorBase {
   public:
-    explicit EnactorContractExpand(bool debug = false) : EnactorBase(BFS_CONTRACT_EXPAND, debug) {}
+    explicit EnactorContractExpand(bool debug = false) : EnactorBase(BFS_CONTRACT_EXPAND, debug, INSTRUMENT) {}
     template <typename Problem>
     bfs_status EnactFusedSearch(Problem &p, int src, int max_grid_size = 0)
     {
@@ -145,7 +155,7 @@ class EnactorContractExpand : public EnactorBase {
 template <bool INSTRUMENT>
 class EnactorTwoPhase : public EnactorBase {
   public:
-    explicit EnactorTwoPhase(bool debug = false) : EnactorBase(BFS_TWO_PHASE, debug) {}
+    explicit EnactorTwoPhase(bool debug = false) : EnactorBase(BFS_TWO_PHASE, debug, INSTRUMENT) {}
     template <typename Problem>
     bfs_status EnactFusedSearch(Problem &p, int src, int max_grid_size = 0)
     {
@@ -163,7 +173,7 @@ class EnactorTwoPhase : public EnactorBase {
 template <bool INSTRUMENT>
 class EnactorDirectionOptimizing : public EnactorBase {
   public:
-    explicit EnactorDirectionOptimizing(bool debug = false) : EnactorBase(BFS_DIRECTION_OPTIMIZING, debug) {}
+    explicit EnactorDirectionOptimizing(bool debug = false) : EnactorBase(BFS_DIRECTION_OPTIMIZING, debug, INSTRUMENT) {}
     template <typename Problem>
     bfs_status EnactFusedSearch(Problem &p, int src, int max_grid_size = 0)
     {
@@ -179,7 +189,7 @@ class EnactorDirectionOptimizing : public EnactorBase {
 template <bool INSTRUMENT>
 class EnactorHybrid : public EnactorBase {
   public:
-    explicit EnactorHybrid(bool debug = false) : EnactorBase(BFS_HYBRID, debug) {}
+    explicit EnactorHybrid(bool debug = false) : EnactorBase(BFS_HYBRID, debug, INSTRUMENT) {}
     template <typename Problem>
     bfs_status EnactSearch(Problem &p, int src, int max_grid_size = 0)
     {

@@ -22,6 +22,7 @@
 #include <stdint.h>
 
 #include "b40c_bfs.h"
+#include "b40c_graph.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -46,6 +47,10 @@ bfs_status bfs_pool_create(int32_t device, bfs_graph_pool **out);
 /* Copies a CSR to the device; returns its id (0, 1, 2, ...). */
 bfs_status bfs_pool_add_graph(bfs_graph_pool *pool, const int32_t *h_row_offsets, const int32_t *h_column_indices,
                               int32_t nodes, int64_t edges, int32_t *graph_id);
+/* Adds a CSR already in device memory (b40c_device_csr_* builders), copied
+ * device to device; the in-neighbour CSR of an asymmetric graph is built on
+ * the device too.  The caller keeps (and frees) its device graph. */
+bfs_status bfs_pool_add_device_graph(bfs_graph_pool *pool, const b40c_device_csr *graph, int32_t *graph_id);
 bfs_status bfs_pool_graph_info(const bfs_graph_pool *pool, int32_t graph_id, int32_t *nodes, int64_t *edges);
 bfs_status bfs_pool_device_bytes(const bfs_graph_pool *pool, int64_t *bytes);
 void bfs_pool_destroy(bfs_graph_pool *pool);

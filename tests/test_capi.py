@@ -53,3 +53,4 @@ def test_invalid_arguments_fail_without_device(lib):
     cfg = _native.BfsConfig()
     assert lib.bfs_config_default(ctypes.byref(cfg)) == 0
     assert cfg.wave_gather_threshold > 0 and cfg.cta_gather_threshold >= cfg.wave_gather_threshold
+

@@ -332,13 +332,33 @@ def test_config1_grid2d_1000(gpu):
 
 @pytest.mark.slow
 def test_config2_rmat22(gpu):
+    # every variant on 8 of the 64 bench sources (one oracle run per source)
     g = CsrGraph.rmat(22, 16)
+    row, col = g.row_offsets, g.column_indices
     p = CsrProblem().from_host_problem(g)
     srcs = g.sample_sources(64)
-    for i, (strategy, fused) in enumerate([(b40c.EXPAND_CONTRACT, True), (b40c.HYBRID, True),
-                                           (b40c.TWO_PHASE, False), (b40c.CONTRACT_EXPAND, True),
-                                           (b40c.DIRECTION_OPTIMIZING, True), (b40c.DIRECTION_OPTIMIZING, False)]):
-        assert_parity(g, p, strategy, fused, srcs[i])
+    for src in srcs[::8]:
+        expect, levels, _ = oracle.bfs(row, col, int(src))
+        for strategy, fused in VARIANTS:
+            got, st = run(p, strategy, fused, src)
+            bad = oracle.compare(got, expect)
+            assert bad == -1, (strategy, fused, int(src), bad)
+            assert st.search_depth == levels
+    p.destroy()
+
+
+@pytest.mark.slow
+def test_config2_rmat22_predecessors(gpu):
+    g = CsrGraph.rmat(22, 16)
+    row, col = g.row_offsets, g.column_indices
+    p = CsrProblem(mark_predecessors=True).from_host_problem(g)
+    for src in g.sample_sources(64)[1::16]:
+        expect, _, _ = oracle.bfs(row, col, int(src))
+        for strategy, fused in [(b40c.EXPAND_CONTRACT, True), (b40c.DIRECTION_OPTIMIZING, True),
+                                (b40c.HYBRID, True), (b40c.CONTRACT_EXPAND, True)]:
+            got, _ = run(p, strategy, fused, src)
+            assert oracle.compare(got, expect) == -1, (strategy, int(src))
+            assert oracle.check_preds(row, col, expect, p.extract_results()) == -1, (strategy, int(src))
     p.destroy()
 
 
@@ -353,8 +373,8 @@ def test_config3_grid3d_300(gpu):
     expect = (i + jj + kk).astype(np.int32)
     del idx, i, rem, jj, kk
     p = CsrProblem().from_host_problem(g)
-    for strategy, fused in [(b40c.EXPAND_CONTRACT, True), (b40c.HYBRID, True)]:
+    for strategy, fused in VARIANTS:
         got, st = run(p, strategy, fused, 0)
-        assert oracle.compare(got, expect) == -1
+        assert oracle.compare(got, expect) == -1, (strategy, fused)
         assert st.search_depth == 898
     p.destroy()
