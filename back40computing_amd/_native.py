@@ -61,7 +61,7 @@ class BfsConfig(ctypes.Structure):
         ("grab_strided", ctypes.c_int32),
         ("claim_filter", ctypes.c_int32),
         ("spread_steps", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 1),
+        ("compaction", ctypes.c_int32),
     ]
 
 

@@ -63,7 +63,7 @@ const bfs_config kDefaults = {
     /*grab_strided*/ 1,
     /*claim_filter*/ 1,
     /*spread_steps*/ 1,
-    {0}};
+    /*compaction*/ 1};
 
 constexpr unsigned kHubCapacity = 1u << 16;  // grid-tier entries per step list
 
@@ -89,6 +89,11 @@ struct bfs_problem {
     int *d_q[2] = {nullptr, nullptr};
     int *d_pq[2] = {nullptr, nullptr};
     size_t q_cap = 0;            // allocated elements per queue
+    // chained compaction: one status word per kBlock-item tile of a queue,
+    // and the epoch base of the next search (advanced by a stride that
+    // exceeds any search's step count; the words are cleared on wrap)
+    unsigned long long *d_tile_status = nullptr;
+    unsigned scan_epoch = 1;
     unsigned cap = 0;            // capacity in use for the current frontier type
     int frontier_type = -1;      // set by reset
     bool dirty = false;          // labels written since the last reset
@@ -142,6 +147,8 @@ SearchArgs search_args(const bfs_problem *p, int grid)
     s.fbm[0] = p->d_fbm;
     s.fbm[1] = p->d_fbm + words;
     s.marks = p->d_marks;
+    s.tile_status = p->cfg.compaction == 2 ? p->d_tile_status : nullptr;
+    s.scan_epoch = p->scan_epoch;
     s.nodes = p->nodes;
     s.total_edges = (unsigned long long) p->edges;
     return s;
@@ -198,6 +205,13 @@ bfs_status enact_hybrid(bfs_problem *p, const GraphArgs &g, const SearchArgs &s,
                         int *launches)
 {
     const bool exact = p->cfg.cull_mode == BFS_CULL_EXACT;
+    // Two-phase levels per host check: the launches of a chunk past the
+    // hand-back (or past an empty frontier) return at once, so the host
+    // waits once per chunk rather than once per level
+    // (enactor_hybrid.cuh:483-492 runs ahead of its done flag the same way).
+    const int chunk = 4;
+    SearchArgs ts = s;
+    ts.hybrid_steps = 1;
     Cursor c{0, 0, kPhaseContractExpand};
     while (true) {
         // Persistent contract-expand until the edge frontier saturates
@@ -211,26 +225,30 @@ bfs_status enact_hybrid(bfs_problem *p, const GraphArgs &g, const SearchArgs &s,
         if (h.exit_reason != kExitSaturated) return BFS_OK;
         // Two-phase levels: filter -> contract -> expand, until the filtered
         // edge frontier fits the persistent kernel again
-        // (enactor_hybrid.cuh:386-532).
+        // (enactor_hybrid.cuh:386-532); the contract step that finds it so
+        // records where and stops the rest of the chunk.
         c = Cursor{h.exit_step, h.exit_iteration, kPhaseFilter};
+        HIP_TRY(hipMemsetAsync(&p->d_ctl->hybrid_stop, 0, sizeof(unsigned), p->stream), "clear hand-back flag");
         while (true) {
-            HIP_TRY(launch_step(kPhaseFilter, p->mark_pred, exact, g, s, c, sgrid, p->stream, launches),
-                    "launch filter");
+            for (int k = 0; k < chunk; k++) {
+                HIP_TRY(launch_step(kPhaseFilter, p->mark_pred, exact, g, ts, c, sgrid, p->stream, launches),
+                        "launch filter");
+                Cursor cc{c.step + 1, c.iteration, kPhaseContract};
+                HIP_TRY(launch_step(kPhaseContract, p->mark_pred, exact, g, ts, cc, sgrid, p->stream, launches),
+                        "launch contract");
+                Cursor ce{c.step + 2, c.iteration, kPhaseExpand};
+                HIP_TRY(launch_step(kPhaseExpand, p->mark_pred, exact, g, ts, ce, sgrid, p->stream, launches),
+                        "launch expand");
+                c = Cursor{c.step + 3, c.iteration + 1, kPhaseFilter};
+            }
             st = read_control(p);
             if (st != BFS_OK) return st;
-            unsigned n = p->h_ctl->qlen[(c.step + 1) % 3];
-            if (p->h_ctl->overflow || n == 0) return BFS_OK;
-            if (n <= (unsigned) s.sat_t) {
-                c = Cursor{c.step + 1, c.iteration, kPhaseContractExpand};
+            if (p->h_ctl->overflow) return BFS_OK;
+            if (p->h_ctl->hybrid_stop) {
+                c = Cursor{p->h_ctl->exit_step, p->h_ctl->exit_iteration, kPhaseContractExpand};
                 break;
             }
-            Cursor cc{c.step + 1, c.iteration, kPhaseContract};
-            HIP_TRY(launch_step(kPhaseContract, p->mark_pred, exact, g, s, cc, sgrid, p->stream, launches),
-                    "launch contract");
-            Cursor ce{c.step + 2, c.iteration, kPhaseExpand};
-            HIP_TRY(launch_step(kPhaseExpand, p->mark_pred, exact, g, s, ce, sgrid, p->stream, launches),
-                    "launch expand");
-            c = Cursor{c.step + 3, c.iteration + 1, kPhaseFilter};
+            if (p->h_ctl->qlen[c.step % 3] == 0) return BFS_OK;
         }
     }
 }
@@ -331,6 +349,8 @@ void free_queues(bfs_problem *p)
         if (p->d_pq[i]) (void) hipFree(p->d_pq[i]);
         p->d_q[i] = p->d_pq[i] = nullptr;
     }
+    if (p->d_tile_status) (void) hipFree(p->d_tile_status);
+    p->d_tile_status = nullptr;
     p->q_cap = 0;
 }
 
@@ -487,6 +507,11 @@ bfs_status bfs_problem_reset(bfs_problem *p, bfs_frontier_type ft, double queue_
             HIP_TRY(hipMalloc(&p->d_q[i], sizeof(int) * cap), "allocate frontier queue");
             if (p->mark_pred) HIP_TRY(hipMalloc(&p->d_pq[i], sizeof(int) * cap), "allocate predecessor queue");
         }
+        const size_t words = cap / kBlock + 2;
+        HIP_TRY(hipMalloc(&p->d_tile_status, sizeof(unsigned long long) * words), "allocate tile status words");
+        HIP_TRY(hipMemsetAsync(p->d_tile_status, 0, sizeof(unsigned long long) * words, p->stream),
+                "clear tile status words");
+        p->scan_epoch = 1;
         p->q_cap = cap;
     }
     p->cap = (unsigned) cap;
@@ -509,6 +534,8 @@ bfs_status bfs_problem_set_config(bfs_problem *p, const bfs_config *cfg)
     if (c.grid_gather_threshold < 0) c.grid_gather_threshold = -1;  // grid tier off
     if (c.claim_filter == 0) c.claim_filter = kDefaults.claim_filter;
     if (c.spread_steps == 0) c.spread_steps = kDefaults.spread_steps;
+    if (c.compaction == 0) c.compaction = kDefaults.compaction;
+    if (c.compaction < 1 || c.compaction > 2) return fail(BFS_ERR_INVALID_ARG, "unknown compaction mode");
     if (c.do_alpha == 0) c.do_alpha = kDefaults.do_alpha;
     if (c.do_beta == 0) c.do_beta = kDefaults.do_beta;
     if (c.do_alpha < 1 || c.do_beta < 1) return fail(BFS_ERR_INVALID_ARG, "invalid direction switch points");
@@ -553,7 +580,18 @@ bfs_status bfs_enact(bfs_problem *p, bfs_strategy strategy, int32_t fused, int32
         return fail(BFS_ERR_LAUNCH, "grid exceeds co-resident capacity");
     int sgrid = step_grid(p, max_grid_size);
     GraphArgs g{p->d_row, p->d_col, p->nodes, p->d_rrow, p->d_rcol};
+    // Chained compaction tags each step's tile words with scan_epoch + step.
+    // A search has fewer than 3 (V + 2) steps (every level but the last
+    // labels a vertex; two-phase levels take three steps).
+    const unsigned long long stride = 3ull * ((unsigned long long) p->nodes + 2);
+    if (p->cfg.compaction == 2 && (unsigned long long) p->scan_epoch + 2 * stride >= (1ull << 30)) {
+        const size_t words = p->q_cap / kBlock + 2;
+        HIP_TRY(hipMemsetAsync(p->d_tile_status, 0, sizeof(unsigned long long) * words, p->stream),
+                "clear tile status words");
+        p->scan_epoch = 1;
+    }
     SearchArgs s = search_args(p, pgrid);
+    if (p->cfg.compaction == 2) p->scan_epoch += (unsigned) stride;
     if (strategy != BFS_HYBRID) s.sat_t = -1;
 
     int launches = 0;
@@ -858,9 +896,14 @@ void apply_search_config(SearchArgs &s, const bfs_config &cfg, int grid)
     s.coherent_filter = cfg.claim_filter == 2;
     s.spread = cfg.spread_steps == 1;
     s.fast_t = cfg.fast_path_threshold;
+    // the reference leaves at grid * 512 items for 128-thread CTAs
+    // (enactor_hybrid.cuh:571): four items per thread of the grid
     s.sat_t = cfg.saturation_threshold > 0 ? cfg.saturation_threshold
-              : (cfg.saturation_threshold < 0 ? -1 : grid * 256);
+              : (cfg.saturation_threshold < 0 ? -1 : grid * kBlock * 4);
+    s.hybrid_steps = 0;
     s.instrument = cfg.instrument;
+    s.tile_status = nullptr;
+    s.scan_epoch = 0;
     s.track_edges = 1;
     s.do_alpha = cfg.do_alpha;
     s.do_beta = cfg.do_beta;

@@ -46,7 +46,9 @@ struct alignas(256) Control {
     unsigned long long life_cycles;   // instrumented: sum over workgroups of kernel time
     unsigned long long hub_alloc[2];  // grid tier: (entries << 32) | chunks, by step & 1
     unsigned hub_steal[2];            // grid tier: chunk counter, by step & 1
-    unsigned long long pad1[2];
+    unsigned hybrid_stop;             // hybrid two-phase launches: the filtered frontier fits the
+                                      // persistent kernel again (exit_step / exit_iteration say where)
+    unsigned pad1[3];
     // Direction-optimizing bookkeeping (second 128-byte line).
     unsigned long long fedges[3];     // sum of degrees of each frontier, by step % 3
     unsigned long long visited_edges; // sum of degrees of all vertices labelled so far

@@ -46,10 +46,19 @@ struct SearchArgs {
     int track_edges;       // accumulate frontier edge counts (direction switching)
     int do_alpha, do_beta; // top-down -> bottom-up when m_f * alpha > m_u; back when n_f * beta < V
     int wave_t, cta_t, grid_t, fast_t, sat_t;
+    // Hybrid two-phase launches: a contract step whose input is at most
+    // sat_t hands the search back to the persistent contract-expand kernel
+    // (sets Control::hybrid_stop), and every step launched after it returns.
+    int hybrid_steps;
     int instrument;
     int grab_max;          // most tiles taken per tile-counter atomic
     int coherent_filter;   // claim pre-test reads the device-coherent visited word
     int spread;            // few-member top-down levels split their edges evenly over the grid
+    // Chained compaction of contract and filter steps: one status word per
+    // tile, tagged with epoch scan_epoch + step (never 0: cleared words read
+    // as unpublished).  Null tile_status: wave-staged atomic reservation.
+    unsigned long long *tile_status;
+    unsigned scan_epoch;
     // Search start folded into a persistent launch (init_src >= 0): the
     // kernel clears the status head, queues init_src, labels it
     // (init_label) and, with init_reset, rewrites every label first --

@@ -19,6 +19,7 @@
 // Each step runs either inside a persistent, cooperatively launched loop with
 // in-kernel grid barriers (the reference's KernelGlobalBarrier kernels) or
 // as kernel launches per step.
+#include "b40c_primitives.hpp"
 #include "bfs_device.hpp"
 #include "bfs_internal.hpp"
 
@@ -645,6 +646,66 @@ __device__ void body_expand(const GraphArgs &g, const SearchArgs &s, Cursor c, u
     finish_step<PRED>(wq, lds, s.ctl, c.step, false, 0, 0u, false);
 }
 
+// ---------------------------------------------------------------------------
+// Chained compaction (config compaction = 2) of the steps whose output per
+// tile is bounded by the tile (contract: claim, filter: bitmask cull).  The
+// tile's keep flags are ranked by a workgroup scan (wave DPP sums + LDS wave
+// totals); the tile's offset in the output queue is the sum of all earlier
+// tiles' counts, found by a decoupled look-back over one status word per
+// tile (prim::lookback_prefix) -- a single-pass grid scan in place of the
+// reference's per-tile atomicAdd reservation (util/scan/cooperative_scan.cuh:
+// 309-343) and its multi-kernel scan passes.  The output keeps the input
+// order.  Tiles are handed out in increasing order by the step's tile
+// counter, so a tile only waits for tiles already taken; a workgroup's next
+// tile of the same grab continues from its own running prefix without a
+// look-back.  The last tile stores the output length.
+// ---------------------------------------------------------------------------
+struct ChainCarry {
+    unsigned next;   // tile that continues this workgroup's running prefix
+    unsigned carry;  // inclusive prefix through tile next - 1
+};
+
+template <bool PRED>
+__device__ __forceinline__ void chained_append(const SearchArgs &s, Cursor c, unsigned n, unsigned tile, bool keep,
+                                               int u, int p, Lds<PRED> &lds, ChainCarry &cc)
+{
+    int total;
+    const int before = block_exclusive_sum(keep ? 1 : 0, lds.ep_scan, &total);
+    const unsigned epoch = (s.scan_epoch + (unsigned) c.step) & 0x3FFFFFFFu;
+    if (threadIdx.x < kWave) {
+        unsigned prefix = 0;
+        if (tile == 0) {
+            if (threadIdx.x == 0) prim::publish(s.tile_status, 0, epoch, prim::kStatePrefix, (unsigned) total);
+        } else if (tile == cc.next) {
+            prefix = cc.carry;
+            if (threadIdx.x == 0)
+                prim::publish(s.tile_status, tile, epoch, prim::kStatePrefix, prefix + (unsigned) total);
+        } else {
+            if (threadIdx.x == 0) prim::publish(s.tile_status, tile, epoch, prim::kStateAggregate, (unsigned) total);
+            prefix = prim::lookback_prefix(s.tile_status, tile, epoch);
+            if (threadIdx.x == 0)
+                prim::publish(s.tile_status, tile, epoch, prim::kStatePrefix, prefix + (unsigned) total);
+        }
+        if (threadIdx.x == 0) lds.ep_base = prefix;
+    }
+    __syncthreads();
+    const unsigned prefix = lds.ep_base;
+    cc.next = tile + 1;
+    cc.carry = prefix + (unsigned) total;
+    const int step_out = (c.step + 1) & 1;
+    if (keep) {
+        const unsigned at = prefix + (unsigned) before;
+        if (at < s.cap) {
+            s.q[step_out][at] = u;
+            if (PRED) s.pq[step_out][at] = p;
+        } else {
+            st_agent(&s.ctl->overflow, 1u);
+            st_agent(&s.ctl->ovf[(c.step + 1) % 3], 1u);
+        }
+    }
+    if (threadIdx.x == 0 && tile == (n + kBlock - 1) / kBlock - 1) st_agent(&s.ctl->qlen[(c.step + 1) % 3], cc.carry);
+}
+
 template <bool PRED, bool EXACT>
 __device__ void body_contract(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds)
 {
@@ -654,8 +715,10 @@ __device__ void body_contract(const GraphArgs &g, const SearchArgs &s, Cursor c,
     const int *pin = s.pq[c.step & 1];
     bool labeled = false;
     TileGrab tg = tile_grab(s, n);
+    ChainCarry cc{~0u, 0u};
     while (true) {
-        unsigned base = next_tile(&s.ctl->steal[c.step & 1], lds, tg) * kBlock;
+        const unsigned tile = next_tile(&s.ctl->steal[c.step & 1], lds, tg);
+        const unsigned base = tile * kBlock;
         if (base >= n) break;
         unsigned i = base + threadIdx.x;
         int u = -1, p = -1;
@@ -669,7 +732,8 @@ __device__ void body_contract(const GraphArgs &g, const SearchArgs &s, Cursor c,
             if (PRED) s.pred[u] = p;
             labeled = true;
         }
-        wq.push(keep, u, -1);
+        if (s.tile_status) chained_append<PRED>(s, c, n, tile, keep, u, -1, lds, cc);
+        else wq.push(keep, u, -1);
     }
     finish_step<PRED>(wq, lds, s.ctl, c.step, labeled, c.iteration, 0u, false);
 }
@@ -682,8 +746,10 @@ __device__ void body_filter(const GraphArgs &g, const SearchArgs &s, Cursor c, u
     const int *in = s.q[c.step & 1];
     const int *pin = s.pq[c.step & 1];
     TileGrab tg = tile_grab(s, n);
+    ChainCarry cc{~0u, 0u};
     while (true) {
-        unsigned base = next_tile(&s.ctl->steal[c.step & 1], lds, tg) * kBlock;
+        const unsigned tile = next_tile(&s.ctl->steal[c.step & 1], lds, tg);
+        const unsigned base = tile * kBlock;
         if (base >= n) break;
         unsigned i = base + threadIdx.x;
         int u = -1, p = -1;
@@ -699,7 +765,8 @@ __device__ void body_filter(const GraphArgs &g, const SearchArgs &s, Cursor c, u
             unsigned w = s.visited[u >> 5];
             keep = ((w >> (u & 31)) & 1u) == 0;
         }
-        wq.push(keep, u, p);
+        if (s.tile_status) chained_append<PRED>(s, c, n, tile, keep, u, p, lds, cc);
+        else wq.push(keep, u, p);
     }
     finish_step<PRED>(wq, lds, s.ctl, c.step, false, 0, 0u, false);
 }
@@ -1808,8 +1875,20 @@ template <bool PRED, bool EXACT, int PHASE>
 __global__ __launch_bounds__(kBlock) void step_kernel(GraphArgs g, SearchArgs s, Cursor c)
 {
     __shared__ Lds<PRED> lds;
+    if (s.hybrid_steps && ld_agent(&s.ctl->hybrid_stop)) return;
     unsigned n = frontier_length(s, c.step);
     if (ld_agent(&s.ctl->overflow)) n = 0;
+    if (PHASE == kPhaseContract && s.hybrid_steps && n > 0 && n <= (unsigned) s.sat_t) {
+        // The filtered edge frontier fits the persistent kernel again: it
+        // runs this step (enactor_hybrid.cuh:386-532 leaves the two-phase
+        // loop at the same point).  Every workgroup sees the same n.
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            s.ctl->exit_step = c.step;
+            s.ctl->exit_iteration = c.iteration;
+            st_agent(&s.ctl->hybrid_stop, 1u);
+        }
+        return;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) prepare_step(s.ctl, c, n);
     if (n == 0) return;
     init_lds(lds);
@@ -1820,6 +1899,7 @@ template <bool PRED, bool EXACT, int PHASE>
 __global__ __launch_bounds__(kBlock) void hub_kernel(GraphArgs g, SearchArgs s, Cursor c)
 {
     __shared__ Lds<PRED> lds;
+    if (s.hybrid_steps && ld_agent(&s.ctl->hybrid_stop)) return;
     if (!hubs_pending(s, c.step)) return;
     init_lds(lds);
     // the tile step's input length: it sets the chunk size of the list

@@ -96,7 +96,8 @@ typedef struct bfs_config {
     int32_t fast_path_threshold;    /* frontier <= this: one workgroup runs the level
                                        with no grid barrier (-1 disables)             */
     int32_t saturation_threshold;   /* hybrid: edge frontier > this leaves the
-                                       persistent contract-expand kernel (-1: never)   */
+                                       persistent contract-expand kernel (-1: never;
+                                       default: 4 items per thread of the grid)       */
     int32_t blocks_per_cu;          /* persistent grid = CUs * this (capped by occupancy) */
     int32_t cull_mode;              /* bfs_cull_mode */
     int32_t instrument;             /* 1: collect per-workgroup duty statistics */
@@ -121,7 +122,12 @@ typedef struct bfs_config {
                                        too many edges for one workgroup: 1 = every
                                        workgroup walks an equal share of their edges
                                        (one phase), 2 = tile and grid-tier phases */
-    int32_t reserved[1];
+    int32_t compaction;             /* output of contract and filter steps (two-phase,
+                                       hybrid): 1 = each wave stages it in LDS and
+                                       reserves queue space with one atomic per flush;
+                                       2 = a single-pass chained scan over the step's
+                                       tiles (decoupled look-back), which keeps the
+                                       input order */
 } bfs_config;
 
 typedef struct bfs_stats {
