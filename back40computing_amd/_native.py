@@ -177,6 +177,7 @@ def load():
     _sig(lib, "b40c_copy_if", ctypes.c_int, vp, vp, vp, i64, ctypes.c_int, ctypes.c_uint32,
          ctypes.POINTER(i64), vp)
     _sig(lib, "b40c_scan_last_ms", ctypes.c_int, vp, ctypes.POINTER(dbl))
+    _sig(lib, "b40c_scan_noalloc", ctypes.c_int, i32, vp, vp, i64, i32, vp, vp)
 
     _sig(lib, "bfs_status_string", ctypes.c_char_p, ctypes.c_int)
     _sig(lib, "bfs_last_error_message", ctypes.c_char_p)

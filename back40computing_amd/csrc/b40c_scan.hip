@@ -14,6 +14,7 @@
 #include "bfs_internal.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 using namespace b40c_hip;
 
@@ -140,6 +141,12 @@ __global__ __launch_bounds__(kThreads) void copy_if_kernel(const uint32_t *in, u
     if (d_count && tile == c.tiles - 1 && threadIdx.x == 0) *d_count = prefix + agg;
 }
 
+__global__ void fill_u32_kernel(uint32_t *p, long long n, uint32_t value)
+{
+    for (long long i = blockIdx.x * (long long) blockDim.x + threadIdx.x; i < n; i += (long long) gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
 }  // namespace
 
 hipError_t launch_scan_u32(const uint32_t *in, uint32_t *out, long long n, bool exclusive, const ChainArgs &c,
@@ -159,6 +166,48 @@ hipError_t launch_copy_if_u32(const uint32_t *in, uint32_t *out, long long n, in
                        vec, c, d_count);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// Allocation-free scan (b40c_scan_noalloc): the tile counter and the status
+// words live in one file-scope device array per GPU.  Word 0 is the counter;
+// the status words start at word kNoAllocStatus (a 256-byte line of their
+// own).  Every call clears what it uses with a fill kernel (no memset node
+// if the caller captures the call in a graph) and uses epoch 0.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr long long kNoAllocMaxTiles = 16384;  // 2^26 items
+constexpr int kNoAllocStatus = 32;
+constexpr int kNoAllocDevices = 64;
+
+static __device__ __align__(256) unsigned long long scratch[kNoAllocStatus + kNoAllocMaxTiles];
+
+__global__ __launch_bounds__(kThreads) void noalloc_clear_kernel(unsigned long long *words, long long tiles)
+{
+    for (long long i = blockIdx.x * (long long) kThreads + threadIdx.x; i < tiles; i += (long long) gridDim.x * kThreads)
+        words[kNoAllocStatus + i] = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x == 0) words[0] = 0ull;
+}
+
+std::mutex g_noalloc_mu;
+unsigned long long *g_noalloc_ptr[kNoAllocDevices];  // device address of `scratch`, per device
+
+// The device address of this device's `scratch` (the array's name is a host
+// stand-in, not a device address), looked up once per device.
+hipError_t noalloc_scratch(int device, unsigned long long **out)
+{
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[device]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_noalloc_ptr[device] = static_cast<unsigned long long *>(ptr);
+    }
+    *out = g_noalloc_ptr[device];
+    return hipSuccess;
+}
+
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // Chain state: the tile counter and the per-tile status words.
@@ -328,6 +377,42 @@ bfs_status b40c_copy_if(b40c_scan_enactor *e, const uint32_t *d_in, uint32_t *d_
             return host_hip_fail(r, "read compacted count");
         *h_num_compacted = k;
     }
+    return BFS_OK;
+}
+
+bfs_status b40c_scan_noalloc(int32_t device, const uint32_t *d_in, uint32_t *d_out, int64_t n, int32_t exclusive,
+                             uint32_t *d_total, void *stream)
+{
+    if (n < 0 || (n > 0 && (!d_in || !d_out))) return host_fail(BFS_ERR_INVALID_ARG, "invalid scan arguments");
+    if (n > kNoAllocMaxTiles * kScanTile)
+        return host_fail(BFS_ERR_INVALID_ARG, "scan longer than the allocation-free limit of 2^26 items");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return host_fail(BFS_ERR_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= count || device >= kNoAllocDevices)
+        return host_fail(BFS_ERR_INVALID_ARG, "device ordinal out of range");
+    hipError_t r;
+    if ((r = hipSetDevice(device)) != hipSuccess) return host_hip_fail(r, "hipSetDevice");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (n == 0) {
+        if (d_total) {
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, d_total, 1, 0u);
+            if ((r = hipGetLastError()) != hipSuccess) return host_hip_fail(r, "clear scan total");
+        }
+        return BFS_OK;
+    }
+    unsigned long long *words = nullptr;
+    if ((r = noalloc_scratch(device, &words)) != hipSuccess) return host_hip_fail(r, "scan scratch address");
+    ChainArgs c;
+    c.tiles = (n + kScanTile - 1) / kScanTile;
+    c.status = words + kNoAllocStatus;
+    c.tile_counter = words;
+    c.tile_base = 0;
+    c.epoch = 0;
+    const unsigned clear_blocks = (unsigned) std::min<long long>((c.tiles + kThreads - 1) / kThreads, 64);
+    hipLaunchKernelGGL(noalloc_clear_kernel, dim3(clear_blocks), dim3(kThreads), 0, st, words, c.tiles);
+    if ((r = hipGetLastError()) != hipSuccess) return host_hip_fail(r, "launch scratch clear");
+    if ((r = launch_scan_u32(d_in, d_out, n, exclusive != 0, c, d_total, st)) != hipSuccess)
+        return host_hip_fail(r, "launch scan");
     return BFS_OK;
 }
 

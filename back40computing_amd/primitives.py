@@ -55,6 +55,16 @@ class ScanEnactor:
             pass
 
 
+NOALLOC_MAX_ITEMS = 1 << 26
+
+
+def scan_noalloc(d_in, d_out, n, exclusive=True, d_total=None, stream=None, device=0):
+    """b40c_scan_noalloc: the scan with no device allocation (scratch in a
+    static device array, n <= 2^26), queued on `stream` without waiting."""
+    _check(_native.load().b40c_scan_noalloc(device, d_in, d_out, int(n), int(exclusive), d_total, stream),
+           "b40c_scan_noalloc")
+
+
 class DeviceCsr:
     """A CSR graph in device memory, built on the device."""
 

@@ -61,6 +61,20 @@ bfs_status b40c_copy_if(b40c_scan_enactor *e, const uint32_t *d_in, uint32_t *d_
                         b40c_select_op op, uint32_t comparand, int64_t *h_num_compacted,
                         uint32_t *d_num_compacted);
 
+/* addition, no counterpart in the reference: b40c_scan that allocates
+ * nothing, for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  The tile counter and status words
+ * live in a static device array of 128 KiB per GPU, held until the process
+ * ends; n may be at most 2^26 (67,108,864) items, larger calls return
+ * BFS_ERR_INVALID_ARG.  Asynchronous: the clear and scan kernels are queued
+ * on `stream` (a hipStream_t; null = the default stream) and the call
+ * returns without waiting.  The sum of all n items goes to *d_total (device,
+ * optional).  Every call on one GPU shares that GPU's array: calls that
+ * overlap in time on one GPU (other streams or host threads) race and give
+ * wrong results; calls queued on one stream are ordered and safe. */
+bfs_status b40c_scan_noalloc(int32_t device, const uint32_t *d_in, uint32_t *d_out, int64_t n, int32_t exclusive,
+                             uint32_t *d_total, void *stream);
+
 /* addition: elapsed device time of the enactor's last call (ms, hipEvents). */
 bfs_status b40c_scan_last_ms(const b40c_scan_enactor *e, double *ms);
 

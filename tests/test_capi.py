@@ -54,3 +54,9 @@ def test_invalid_arguments_fail_without_device(lib):
     assert lib.bfs_config_default(ctypes.byref(cfg)) == 0
     assert cfg.wave_gather_threshold > 0 and cfg.cta_gather_threshold >= cfg.wave_gather_threshold
 
+
+
+def test_noalloc_scan_rejects_oversize_without_device(lib):
+    # the allocation-free scan's 2^26-item limit is checked before any HIP call
+    assert lib.b40c_scan_noalloc(0, None, None, (1 << 26) + 1, 1, None, None) == 1
+    assert lib.b40c_scan_noalloc(0, None, None, -1, 1, None, None) == 1
