@@ -48,6 +48,18 @@ SPACE_SINGLE = {
     "spread_steps": [1, 2],
 }
 
+# High-diameter lattices (one search of hundreds to thousands of levels):
+# the single-workgroup fast path, the grid shape and the tile grabs matter,
+# the direction switch points do not (the search stays top-down).
+SPACE_LATTICE = {
+    "blocks_per_cu": [1, 2],
+    "fast_path_threshold": [256, 512, 1024],
+    "tile_grab_max": [1, 4, 16, 32],
+    "wave_gather_threshold": [8, 32, 64],
+    "cta_gather_threshold": [256, 512, 1024],
+    "compaction": [1, 2],
+}
+
 SPACE_BATCH = {
     "blocks_per_cu": [1, 2, 4],
     "tile_grab_max": [1, 8, 32, 128],
@@ -67,7 +79,7 @@ def default_config(kind="single"):
     cfg = _native.BfsConfig()
     lib = _native.load()
     (lib.bfs_batch_config_default if kind == "batch" else lib.bfs_config_default)(ctypes.byref(cfg))
-    return {f: getattr(cfg, f) for f, _ in _native.BfsConfig._fields_ if f != "reserved"}
+    return {f: getattr(cfg, f) for f, _ in _native.BfsConfig._fields_}
 
 
 def device_name(device=0):
@@ -140,6 +152,30 @@ def tune_single(problem, sources, strategy=4, reps=3, space=None, passes=3, log=
     return coordinate_descent(measure, default_config("single"), space, passes, log)
 
 
+def tune_lattice(problems, strategy=1, reps=3, space=None, passes=3, log=None):
+    """Tunes one configuration for several lattice problems (source 0 each).
+
+    Time = median over reps of the summed traversal time; signature = CRC
+    of every problem's depth labels."""
+    space = space or SPACE_LATTICE
+
+    def measure(cfg):
+        sig = 0
+        runs = []
+        for r in range(reps):
+            tot = 0.0
+            for p in problems:
+                p.set_config(**cfg)
+                p.reset(1, 1.3)
+                tot += p.enact(strategy, 0).elapsed_ms
+                if r == 0:
+                    sig = zlib.crc32(p.extract_depths().tobytes(), sig)
+            runs.append(tot)
+        return _median(runs), sig
+
+    return coordinate_descent(measure, default_config("single"), space, passes, log)
+
+
 def tune_batch(batch, graph_ids, sources, mode=0, reps=3, space=None, passes=3, check=None, log=None):
     """Tunes a Batch over one set of queries.
 
@@ -185,7 +221,7 @@ def save_entry(kind, cfg, ms, workload, path=TABLE_PATH, device=None):
 
 
 def tuned_config(kind, path=TABLE_PATH, device=None):
-    """The tuned bfs_config fields for `kind` ("single" or "batch") on this
+    """The tuned bfs_config fields for `kind` ("single", "lattice" or "batch") on this
     device, or {} when the table has none (library defaults apply)."""
     table = load_table(path)
     dev = device or device_name()
@@ -204,7 +240,7 @@ def main(argv=None):
     import sys
 
     ap = argparse.ArgumentParser(description="tune bfs_config on this device")
-    ap.add_argument("--kind", choices=["single", "batch"], required=True)
+    ap.add_argument("--kind", choices=["single", "lattice", "batch"], required=True)
     ap.add_argument("--scale", type=int, default=22, help="single: R-MAT scale")
     ap.add_argument("--edge-factor", type=int, default=16, help="single: R-MAT edge factor")
     ap.add_argument("--sources", type=int, default=8, help="single: sources per measurement")
@@ -225,6 +261,14 @@ def main(argv=None):
         best, ms, trials = tune_single(p, srcs, reps=a.reps, passes=a.passes, log=log)
         workload = f"rmat{a.scale}_ef{a.edge_factor}_{a.sources}src direction-optimizing"
         p.destroy()
+    elif a.kind == "lattice":
+        from .bfs import CsrGraph, CsrProblem
+        graphs = [CsrGraph.grid2d(1000), CsrGraph.grid3d(300)]
+        problems = [CsrProblem().from_host_problem(g) for g in graphs]
+        best, ms, trials = tune_lattice(problems, reps=a.reps, passes=a.passes, log=log)
+        workload = "grid2d 1000^2 + grid3d 300^3 from vertex 0, contract-expand"
+        for p in problems:
+            p.destroy()
     else:
         from .batch import Batch, GraphPool
         from .workloads import batched_rmat

@@ -227,7 +227,7 @@ class CsrProblem:
     def get_config(self):
         cfg = _native.BfsConfig()
         _check(_native.load().bfs_problem_get_config(self._h, ctypes.byref(cfg)), "get_config")
-        return {f: getattr(cfg, f) for f, _ in _native.BfsConfig._fields_ if f != "reserved"}
+        return {f: getattr(cfg, f) for f, _ in _native.BfsConfig._fields_}
 
     def enact(self, strategy, src, fused=True, max_grid_size=0):
         st = _native.BfsStats()

@@ -649,61 +649,108 @@ __device__ void body_expand(const GraphArgs &g, const SearchArgs &s, Cursor c, u
 // ---------------------------------------------------------------------------
 // Chained compaction (config compaction = 2) of the steps whose output per
 // tile is bounded by the tile (contract: claim, filter: bitmask cull).  The
-// tile's keep flags are ranked by a workgroup scan (wave DPP sums + LDS wave
-// totals); the tile's offset in the output queue is the sum of all earlier
-// tiles' counts, found by a decoupled look-back over one status word per
-// tile (prim::lookback_prefix) -- a single-pass grid scan in place of the
-// reference's per-tile atomicAdd reservation (util/scan/cooperative_scan.cuh:
-// 309-343) and its multi-kernel scan passes.  The output keeps the input
-// order.  Tiles are handed out in increasing order by the step's tile
-// counter, so a tile only waits for tiles already taken; a workgroup's next
-// tile of the same grab continues from its own running prefix without a
-// look-back.  The last tile stores the output length.
+// unit of the chain is a grab (the consecutive tiles one tile-counter atomic
+// hands a workgroup).  Pass 1 decides every item of the grab and keeps one
+// ballot mask per wave and tile in LDS; wave 0 sums the masks, publishes
+// the grab's count in its 64-bit status word (epoch, state, value) and finds
+// the grab's offset in the output queue by a decoupled look-back over the
+// earlier grabs' words (prim::lookback_prefix) -- a single-pass grid scan in
+// place of the reference's per-tile atomicAdd reservation
+// (util/scan/cooperative_scan.cuh:309-343) and its multi-kernel scan passes.
+// Pass 2 re-reads the kept items (still in cache) and writes them at the
+// offset plus their rank, so the output keeps the input order.  Grabs are
+// taken in increasing order and publish their count before looking back,
+// so a grab waits only for grabs already taken, each at most one pass 1
+// away.  The last grab stores the output length.
 // ---------------------------------------------------------------------------
-struct ChainCarry {
-    unsigned next;   // tile that continues this workgroup's running prefix
-    unsigned carry;  // inclusive prefix through tile next - 1
-};
+constexpr unsigned kChainGrabMax = kWave;  // tiles per grab: one lane each in the count scan
+static_assert(kChainGrabMax * kWaves * 2 <= kWaves * kWaveQueueCap, "grab masks fit the staging area");
 
-template <bool PRED>
-__device__ __forceinline__ void chained_append(const SearchArgs &s, Cursor c, unsigned n, unsigned tile, bool keep,
-                                               int u, int p, Lds<PRED> &lds, ChainCarry &cc)
+template <bool PRED, bool EXACT, bool FILTER>
+__device__ __forceinline__ void chained_step(const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds,
+                                             bool &labeled)
 {
-    int total;
-    const int before = block_exclusive_sum(keep ? 1 : 0, lds.ep_scan, &total);
+    const int *in = s.q[c.step & 1];
+    const int *pin = s.pq[c.step & 1];
+    int *out = s.q[(c.step + 1) & 1];
+    int *pout = PRED ? s.pq[(c.step + 1) & 1] : nullptr;
+    // the wave-queue staging is idle: ballot masks [tile][wave]
+    unsigned long long *masks = reinterpret_cast<unsigned long long *>(&lds.wq[0][0]);
+    int *tile_off = &lds.sc_excl[0][0];
+    const int lane = lane_id(), wave = threadIdx.x / kWave;
+    const unsigned tiles = (n + kBlock - 1) / kBlock;
+    const unsigned g = min(tile_grab(s, n).grab, kChainGrabMax);
+    const unsigned grabs = (tiles + g - 1) / g;
     const unsigned epoch = (s.scan_epoch + (unsigned) c.step) & 0x3FFFFFFFu;
-    if (threadIdx.x < kWave) {
-        unsigned prefix = 0;
-        if (tile == 0) {
-            if (threadIdx.x == 0) prim::publish(s.tile_status, 0, epoch, prim::kStatePrefix, (unsigned) total);
-        } else if (tile == cc.next) {
-            prefix = cc.carry;
-            if (threadIdx.x == 0)
-                prim::publish(s.tile_status, tile, epoch, prim::kStatePrefix, prefix + (unsigned) total);
-        } else {
-            if (threadIdx.x == 0) prim::publish(s.tile_status, tile, epoch, prim::kStateAggregate, (unsigned) total);
-            prefix = prim::lookback_prefix(s.tile_status, tile, epoch);
-            if (threadIdx.x == 0)
-                prim::publish(s.tile_status, tile, epoch, prim::kStatePrefix, prefix + (unsigned) total);
+    while (true) {
+        const unsigned first = next_item(&s.ctl->steal[c.step & 1], lds, g);
+        if (first >= tiles) break;
+        const unsigned cnt = min(g, tiles - first);
+        for (unsigned k = 0; k < cnt; k++) {
+            const unsigned i = (first + k) * kBlock + threadIdx.x;
+            bool keep = false;
+            if (i < n) {
+                const int u = ld_queue(in + i);
+                if (FILTER) {
+                    if (u >= 0) keep = ((s.visited[u >> 5] >> (u & 31)) & 1u) == 0;
+                } else {
+                    const int p = PRED ? ld_queue(pin + i) : -1;
+                    keep = u >= 0 && history_pass<PRED>(lds, u) && claim<EXACT>(s.visited, s.depth, u, s.coherent_filter);
+                    if (keep) {
+                        s.depth[u] = c.iteration;
+                        if (PRED) s.pred[u] = p;
+                        labeled = true;
+                    }
+                }
+            }
+            const unsigned long long m = ballot(keep);
+            if (lane == 0) masks[k * kWaves + wave] = m;
         }
-        if (threadIdx.x == 0) lds.ep_base = prefix;
-    }
-    __syncthreads();
-    const unsigned prefix = lds.ep_base;
-    cc.next = tile + 1;
-    cc.carry = prefix + (unsigned) total;
-    const int step_out = (c.step + 1) & 1;
-    if (keep) {
-        const unsigned at = prefix + (unsigned) before;
-        if (at < s.cap) {
-            s.q[step_out][at] = u;
-            if (PRED) s.pq[step_out][at] = p;
-        } else {
+        __syncthreads();
+        if (wave == 0) {
+            int cnt_l = 0;
+            if ((unsigned) lane < cnt)
+                for (int w = 0; w < kWaves; w++) cnt_l += __popcll(masks[lane * kWaves + w]);
+            const int incl = wave_inclusive_sum(cnt_l);
+            if ((unsigned) lane < cnt) tile_off[lane] = incl - cnt_l;
+            const unsigned agg = (unsigned) readlane(incl, kWave - 1);
+            const unsigned j = first / g;
+            unsigned prefix = 0;
+            if (j == 0) {
+                if (lane == 0) prim::publish(s.tile_status, 0, epoch, prim::kStatePrefix, agg);
+            } else {
+                if (lane == 0) prim::publish(s.tile_status, j, epoch, prim::kStateAggregate, agg);
+                prefix = prim::lookback_prefix(s.tile_status, j, epoch);
+                if (lane == 0) prim::publish(s.tile_status, j, epoch, prim::kStatePrefix, prefix + agg);
+            }
+            if (lane == 0) {
+                lds.ep_base = prefix;
+                if (j == grabs - 1) st_agent(&s.ctl->qlen[(c.step + 1) % 3], prefix + agg);
+            }
+        }
+        __syncthreads();
+        const unsigned prefix = lds.ep_base;
+        bool lost = false;
+        for (unsigned k = 0; k < cnt; k++) {
+            const unsigned long long m = masks[k * kWaves + wave];
+            if (!((m >> lane) & 1ull)) continue;
+            unsigned rank = (unsigned) tile_off[k] + (unsigned) __popcll(m & ((1ull << lane) - 1ull));
+            for (int w = 0; w < wave; w++) rank += (unsigned) __popcll(masks[k * kWaves + w]);
+            const unsigned i = (first + k) * kBlock + threadIdx.x;
+            const unsigned at = prefix + rank;
+            if (at < s.cap) {
+                out[at] = ld_queue(in + i);
+                if (PRED) pout[at] = FILTER ? ld_queue(pin + i) : -1;
+            } else {
+                lost = true;
+            }
+        }
+        if (lost) {
             st_agent(&s.ctl->overflow, 1u);
             st_agent(&s.ctl->ovf[(c.step + 1) % 3], 1u);
         }
+        // (next_item's barrier orders these mask reads before the next grab's writes)
     }
-    if (threadIdx.x == 0 && tile == (n + kBlock - 1) / kBlock - 1) st_agent(&s.ctl->qlen[(c.step + 1) % 3], cc.carry);
 }
 
 template <bool PRED, bool EXACT>
@@ -714,26 +761,27 @@ __device__ void body_contract(const GraphArgs &g, const SearchArgs &s, Cursor c,
     const int *in = s.q[c.step & 1];
     const int *pin = s.pq[c.step & 1];
     bool labeled = false;
-    TileGrab tg = tile_grab(s, n);
-    ChainCarry cc{~0u, 0u};
-    while (true) {
-        const unsigned tile = next_tile(&s.ctl->steal[c.step & 1], lds, tg);
-        const unsigned base = tile * kBlock;
-        if (base >= n) break;
-        unsigned i = base + threadIdx.x;
-        int u = -1, p = -1;
-        if (i < n) {
-            u = ld_queue(in + i);
-            if (PRED) p = ld_queue(pin + i);
+    if (s.tile_status) {
+        chained_step<PRED, EXACT, false>(s, c, n, lds, labeled);
+    } else {
+        TileGrab tg = tile_grab(s, n);
+        while (true) {
+            unsigned base = next_tile(&s.ctl->steal[c.step & 1], lds, tg) * kBlock;
+            if (base >= n) break;
+            unsigned i = base + threadIdx.x;
+            int u = -1, p = -1;
+            if (i < n) {
+                u = ld_queue(in + i);
+                if (PRED) p = ld_queue(pin + i);
+            }
+            bool keep = u >= 0 && history_pass<PRED>(lds, u) && claim<EXACT>(s.visited, s.depth, u, s.coherent_filter);
+            if (keep) {
+                s.depth[u] = c.iteration;
+                if (PRED) s.pred[u] = p;
+                labeled = true;
+            }
+            wq.push(keep, u, -1);
         }
-        bool keep = u >= 0 && history_pass<PRED>(lds, u) && claim<EXACT>(s.visited, s.depth, u, s.coherent_filter);
-        if (keep) {
-            s.depth[u] = c.iteration;
-            if (PRED) s.pred[u] = p;
-            labeled = true;
-        }
-        if (s.tile_status) chained_append<PRED>(s, c, n, tile, keep, u, -1, lds, cc);
-        else wq.push(keep, u, -1);
     }
     finish_step<PRED>(wq, lds, s.ctl, c.step, labeled, c.iteration, 0u, false);
 }
@@ -745,28 +793,30 @@ __device__ void body_filter(const GraphArgs &g, const SearchArgs &s, Cursor c, u
     WaveQueue<PRED> wq = make_queue<PRED>(s, c.step, lds);
     const int *in = s.q[c.step & 1];
     const int *pin = s.pq[c.step & 1];
-    TileGrab tg = tile_grab(s, n);
-    ChainCarry cc{~0u, 0u};
-    while (true) {
-        const unsigned tile = next_tile(&s.ctl->steal[c.step & 1], lds, tg);
-        const unsigned base = tile * kBlock;
-        if (base >= n) break;
-        unsigned i = base + threadIdx.x;
-        int u = -1, p = -1;
-        if (i < n) {
-            u = ld_queue(in + i);
-            if (PRED) p = ld_queue(pin + i);
+    if (s.tile_status) {
+        bool unused = false;
+        chained_step<PRED, EXACT, true>(s, c, n, lds, unused);
+    } else {
+        TileGrab tg = tile_grab(s, n);
+        while (true) {
+            unsigned base = next_tile(&s.ctl->steal[c.step & 1], lds, tg) * kBlock;
+            if (base >= n) break;
+            unsigned i = base + threadIdx.x;
+            int u = -1, p = -1;
+            if (i < n) {
+                u = ld_queue(in + i);
+                if (PRED) p = ld_queue(pin + i);
+            }
+            // Bitmask only: the history hash is reserved for claims, because a
+            // hit there must mean the vertex has already been claimed or found
+            // visited by this workgroup.
+            bool keep = false;
+            if (u >= 0) {
+                unsigned w = s.visited[u >> 5];
+                keep = ((w >> (u & 31)) & 1u) == 0;
+            }
+            wq.push(keep, u, p);
         }
-        // Bitmask only: the history hash is reserved for claims, because a
-        // hit there must mean the vertex has already been claimed or found
-        // visited by this workgroup.
-        bool keep = false;
-        if (u >= 0) {
-            unsigned w = s.visited[u >> 5];
-            keep = ((w >> (u & 31)) & 1u) == 0;
-        }
-        if (s.tile_status) chained_append<PRED>(s, c, n, tile, keep, u, p, lds, cc);
-        else wq.push(keep, u, p);
     }
     finish_step<PRED>(wq, lds, s.ctl, c.step, false, 0, 0u, false);
 }
@@ -1237,7 +1287,15 @@ __device__ int fast_scan(FastFront &F, int b, unsigned n)
 // (total edges in all), begs = their row starts, vs = the members.  Claims
 // the unvisited neighbours, labels them, and returns the mask of the slots k
 // that claimed u[k] (parent pv[k], row [ub[k], ud[k])).
-template <bool PRED, bool EXACT>
+//
+// LOCAL: the calling workgroup is the only one touching the labels and the
+// visited words (block 0's fast path, between two grid barriers).  Its
+// claims are then workgroup-scope atomics, done in this XCD's L2, where
+// agent-scope ones must be coherent across the eight XCDs; the barrier
+// before the path invalidated stale lines, and the one after it writes the
+// path's lines back.  Column entries are read through the caches: the next
+// level's rows sit next to this level's on a lattice.
+template <bool PRED, bool EXACT, bool LOCAL>
 __device__ __forceinline__ unsigned fast_edges(const GraphArgs &g, const SearchArgs &s, int next_depth,
                                                const int *offs, const int *begs, const int *vs, unsigned n,
                                                int total, int e0, int (&u)[kFastK], int (&pv)[kFastK],
@@ -1269,7 +1327,8 @@ __device__ __forceinline__ unsigned fast_edges(const GraphArgs &g, const SearchA
         }
     }
 #pragma unroll
-    for (int k = 0; k < kFastK; k++) u[k] = at[k] >= 0 ? __builtin_nontemporal_load(g.col + at[k]) : -1;
+    for (int k = 0; k < kFastK; k++)
+        u[k] = at[k] < 0 ? -1 : LOCAL ? g.col[at[k]] : __builtin_nontemporal_load(g.col + at[k]);
     // visited words (a set bit is final), then claims of the clear ones
     unsigned cur[kFastK];
 #pragma unroll
@@ -1287,11 +1346,17 @@ __device__ __forceinline__ unsigned fast_edges(const GraphArgs &g, const SearchA
     for (int k = 0; k < kFastK; k++) {
         old[k] = ~0u;
         const unsigned bit = 1u << (u[k] & 31);
-        if (u[k] >= 0 && !(cur[k] & bit))
-            // agent scope: another XCD's L2 may hold this word from
-            // before an earlier step's claims
-            old[k] = __hip_atomic_fetch_or(s.visited + (u[k] >> 5), bit, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
+        if (u[k] >= 0 && !(cur[k] & bit)) {
+            // agent scope unless LOCAL: other workgroups claim concurrently,
+            // and another XCD's L2 may hold this word from before an
+            // earlier step's claims
+            if (LOCAL)
+                old[k] = __hip_atomic_fetch_or(s.visited + (u[k] >> 5), bit, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+            else
+                old[k] = __hip_atomic_fetch_or(s.visited + (u[k] >> 5), bit, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
     // (the empty asm keeps the compiler from folding a use into the
     // atomic's own block)
@@ -1340,7 +1405,7 @@ __device__ __forceinline__ void fast_lds_level(const GraphArgs &g, const SearchA
     bool labeled = false;
     for (int b0 = 0; b0 < total; b0 += kBlock * kFastK) {
         int u[kFastK], pv[kFastK], ub[kFastK], ud[kFastK];
-        const unsigned fresh = fast_edges<PRED, EXACT>(g, s, next_depth, F.deg[src], F.beg[src], F.v[src], n, total,
+        const unsigned fresh = fast_edges<PRED, EXACT, true>(g, s, next_depth, F.deg[src], F.beg[src], F.v[src], n, total,
                                                        b0 + (int) threadIdx.x * kFastK, u, pv, ub, ud);
         // append the new members: one LDS reservation per wave
         const int cnt = __popc(fresh);
@@ -1424,7 +1489,7 @@ __device__ __forceinline__ void spread_step(const GraphArgs &g, const SearchArgs
     const int stride = (int) gridDim.x * kBlock * kFastK;
     for (int b0 = (int) blockIdx.x * kBlock * kFastK; b0 < total; b0 += stride) {
         int u[kFastK], pv[kFastK], ub[kFastK], ud[kFastK];
-        const unsigned fresh = fast_edges<PRED, EXACT>(g, s, next_depth, F.deg[0], F.beg[0], F.v[0], n, total,
+        const unsigned fresh = fast_edges<PRED, EXACT, false>(g, s, next_depth, F.deg[0], F.beg[0], F.v[0], n, total,
                                                        b0 + (int) threadIdx.x * kFastK, u, pv, ub, ud);
         int got;
         const int off = block_exclusive_sum(__popc(fresh), F.scan, &got);

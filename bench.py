@@ -110,7 +110,7 @@ def pct(a, q):
     return round(float(np.percentile(np.asarray(a), q)), 4) if len(a) else None
 
 
-def bench_lattices(b40c, np, oracle, tuned_single, cpu_runs):
+def bench_lattices(b40c, np, oracle, tuned, cpu_runs):
     """Configurations 1 and 3: every strategy from vertex 0, median of 5."""
     out = {}
     for name, builder, width in [("grid2d_1000", b40c.CsrGraph.grid2d, 1000),
@@ -119,8 +119,8 @@ def bench_lattices(b40c, np, oracle, tuned_single, cpu_runs):
         g = builder(width)
         deg = g.degrees()
         p = b40c.CsrProblem().from_host_problem(g)
-        if tuned_single:
-            p.set_config(**tuned_single)
+        if tuned:
+            p.set_config(**tuned)
         res = {"vertices": g.nodes, "edges": g.edges, "source": 0, "strategies": {}}
         e_vis = v_vis = None
         for sname, strategy, fused in [("ec", 0, True), ("ce", 1, True), ("tp", 2, True), ("tp_iter", 2, False),
@@ -489,7 +489,9 @@ def main():
 
     lattices = None
     if not args.no_lattices:
-        lattices = bench_lattices(b40c, np, oracle, tuned_single, args.cpu_runs if args.cpu_sample > 0 else 0)
+        # the lattice class of the tuned table when it has one
+        tuned_lattice = {} if args.no_tuned else (autotune.tuned_config("lattice") or tuned_single)
+        lattices = bench_lattices(b40c, np, oracle, tuned_lattice, args.cpu_runs if args.cpu_sample > 0 else 0)
 
     batched = None
     if not args.no_batched:

@@ -56,5 +56,5 @@ def test_table_round_trip(tmp_path):
 def test_search_space_names_are_config_fields():
     from back40computing_amd import _native
     fields = {f for f, _ in _native.BfsConfig._fields_}
-    for space in (autotune.SPACE_SINGLE, autotune.SPACE_BATCH):
+    for space in (autotune.SPACE_SINGLE, autotune.SPACE_LATTICE, autotune.SPACE_BATCH):
         assert set(space) <= fields
