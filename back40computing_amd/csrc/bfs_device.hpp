@@ -135,6 +135,20 @@ __device__ __forceinline__ int wave_inclusive_sum(int x)
     return x;
 }
 
+// Inclusive running maximum of non-negative values across the 64 lanes (the
+// same DPP steps; lanes with no source take 0, the identity here).  The last
+// lane holds the wave's maximum.
+__device__ __forceinline__ int wave_inclusive_max_nonneg(int x)
+{
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false));
+    return x;
+}
+
 // Exclusive prefix sum over the workgroup (one value per thread): wave DPP
 // sums, then the four wave totals through LDS.
 __device__ __forceinline__ int block_exclusive_sum(int x, int *wave_tot, int *total)

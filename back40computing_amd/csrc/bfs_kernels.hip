@@ -1240,16 +1240,23 @@ constexpr int kFastK = 4;       // consecutive edges per thread per batch
 constexpr unsigned long long kFastEdges = 4ull * kBlock * kFastK;
 static_assert(kFastCap <= 4 * kBlock, "the degree scan takes four members per thread");
 
+// Direct levels: when every member's degree is small, thread t takes the
+// whole rows of members t and t + kBlock (up to kFastDirect edges) with no
+// degree scan and no search for each edge's member.
+constexpr int kFastDirect = 8;
+
 struct FastFront {
     int v[2][kFastCap];
     int beg[2][kFastCap];
     int deg[2][kFastCap];        // degree on append; the level's exclusive scan in place
+    int maxdeg[2];               // largest degree of each buffer's members
     unsigned count;              // next-frontier appends
     unsigned long long fe;       // next frontier's edges
     int labeled;
     int scan[kWaves];            // per-wave sums of the degree scan
     unsigned long long wfe[kWaves];
     int wlab[kWaves];
+    int wmax[kWaves];
 };
 
 // The level loop's LDS: the step bodies' staging and the fast path's
@@ -1282,11 +1289,11 @@ __device__ int fast_scan(FastFront &F, int b, unsigned n)
     return total;
 }
 
-// kFastK consecutive edges from e0 (one thread's share of a batch) of an
-// LDS-resident frontier: offs = the exclusive scan of its n members' degrees
-// (total edges in all), begs = their row starts, vs = the members.  Claims
-// the unvisited neighbours, labels them, and returns the mask of the slots k
-// that claimed u[k] (parent pv[k], row [ub[k], ud[k])).
+// Claims of one thread's edge slots: at[k] = column position of slot k (or
+// -1), pv[k] = its row's owner.  Claims the unvisited neighbours, labels
+// them, and returns the mask of the slots k that claimed u[k] (row
+// [ub[k], ud[k])).  Each memory trip (column entry, visited word, claim,
+// row bounds) is issued for every slot before the first result is used.
 //
 // LOCAL: the calling workgroup is the only one touching the labels and the
 // visited words (block 0's fast path, between two grid barriers).  Its
@@ -1295,6 +1302,78 @@ __device__ int fast_scan(FastFront &F, int b, unsigned n)
 // before the path invalidated stale lines, and the one after it writes the
 // path's lines back.  Column entries are read through the caches: the next
 // level's rows sit next to this level's on a lattice.
+template <bool PRED, bool EXACT, bool LOCAL, int S>
+__device__ __forceinline__ unsigned fast_claim_slots(const GraphArgs &g, const SearchArgs &s, int next_depth,
+                                                     const int (&at)[S], const int (&pv)[S], int (&u)[S],
+                                                     int (&ub)[S], int (&ud)[S])
+{
+#pragma unroll
+    for (int k = 0; k < S; k++)
+        u[k] = at[k] < 0 ? -1 : LOCAL ? g.col[at[k]] : __builtin_nontemporal_load(g.col + at[k]);
+    // visited words (a set bit is final), then claims of the clear ones
+    unsigned cur[S];
+#pragma unroll
+    for (int k = 0; k < S; k++) {
+        cur[k] = ~0u;
+        if (u[k] >= 0) {
+            unsigned *w = s.visited + (u[k] >> 5);
+            cur[k] = s.coherent_filter ? __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *w;
+        }
+    }
+    // The claims' results are read only after all are issued (a use
+    // next to its atomic would wait out each round trip in turn).
+    unsigned old[S];
+#pragma unroll
+    for (int k = 0; k < S; k++) {
+        old[k] = ~0u;
+        const unsigned bit = 1u << (u[k] & 31);
+        if (u[k] >= 0 && !(cur[k] & bit)) {
+            // agent scope unless LOCAL: other workgroups claim concurrently,
+            // and another XCD's L2 may hold this word from before an
+            // earlier step's claims
+            if (LOCAL)
+                old[k] = __hip_atomic_fetch_or(s.visited + (u[k] >> 5), bit, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+            else
+                old[k] = __hip_atomic_fetch_or(s.visited + (u[k] >> 5), bit, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    // (the empty asm keeps the compiler from folding a use into the
+    // atomic's own block)
+#pragma unroll
+    for (int k = 0; k < S; k++) asm volatile("" : "+v"(old[k]));
+    unsigned fresh = 0;
+#pragma unroll
+    for (int k = 0; k < S; k++)
+        if (!((old[k] >> (u[k] & 31)) & 1u)) fresh |= 1u << k;
+    if (!EXACT) {
+        // best-effort culling elsewhere may have lost bits of labelled vertices
+        int dep[S];
+#pragma unroll
+        for (int k = 0; k < S; k++) dep[k] = ((fresh >> k) & 1) ? s.depth[u[k]] : -1;
+#pragma unroll
+        for (int k = 0; k < S; k++)
+            if (dep[k] != -1) fresh &= ~(1u << k);
+    }
+#pragma unroll
+    for (int k = 0; k < S; k++) {
+        ub[k] = 0;
+        ud[k] = 0;
+        if ((fresh >> k) & 1) {
+            s.depth[u[k]] = next_depth;
+            if (PRED) s.pred[u[k]] = pv[k];
+            ub[k] = g.row[u[k]];
+            ud[k] = g.row[u[k] + 1];
+        }
+    }
+    return fresh;
+}
+
+// kFastK consecutive edges from e0 (one thread's share of a batch) of an
+// LDS-resident frontier: offs = the exclusive scan of its n members' degrees
+// (total edges in all), begs = their row starts, vs = the members.  Returns
+// fast_claim_slots' mask; pv[k] = the owner of slot k's row.
 template <bool PRED, bool EXACT, bool LOCAL>
 __device__ __forceinline__ unsigned fast_edges(const GraphArgs &g, const SearchArgs &s, int next_depth,
                                                const int *offs, const int *begs, const int *vs, unsigned n,
@@ -1326,72 +1405,49 @@ __device__ __forceinline__ unsigned fast_edges(const GraphArgs &g, const SearchA
             pv[k] = vs[i];
         }
     }
+    return fast_claim_slots<PRED, EXACT, LOCAL, kFastK>(g, s, next_depth, at, pv, u, ub, ud);
+}
+
+// Appends one thread's claimed slots to LDS buffer dst (one LDS reservation
+// per wave); members past kFastCap go to the global queue `spill` at the
+// same positions.  Wave-converged.
+template <int S>
+__device__ __forceinline__ void fast_append(const SearchArgs &s, FastFront &F, int dst, int *spill, unsigned fresh,
+                                            const int (&u)[S], const int (&ub)[S], const int (&ud)[S],
+                                            unsigned long long &fe, bool &labeled, int &dmax)
+{
+    const int cnt = __popc(fresh);
+    const int incl = wave_inclusive_sum(cnt);
+    const int wtot = readlane(incl, kWave - 1);
+    if (!wtot) return;
+    unsigned base = 0;
+    if (lane_id() == 0) base = atomicAdd(&F.count, (unsigned) wtot);
+    unsigned p = (unsigned) readlane((int) base, 0) + (unsigned) (incl - cnt);
 #pragma unroll
-    for (int k = 0; k < kFastK; k++)
-        u[k] = at[k] < 0 ? -1 : LOCAL ? g.col[at[k]] : __builtin_nontemporal_load(g.col + at[k]);
-    // visited words (a set bit is final), then claims of the clear ones
-    unsigned cur[kFastK];
-#pragma unroll
-    for (int k = 0; k < kFastK; k++) {
-        cur[k] = ~0u;
-        if (u[k] >= 0) {
-            unsigned *w = s.visited + (u[k] >> 5);
-            cur[k] = s.coherent_filter ? __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *w;
-        }
-    }
-    // The claims' results are read only after all are issued (a use
-    // next to its atomic would wait out each round trip in turn).
-    unsigned old[kFastK];
-#pragma unroll
-    for (int k = 0; k < kFastK; k++) {
-        old[k] = ~0u;
-        const unsigned bit = 1u << (u[k] & 31);
-        if (u[k] >= 0 && !(cur[k] & bit)) {
-            // agent scope unless LOCAL: other workgroups claim concurrently,
-            // and another XCD's L2 may hold this word from before an
-            // earlier step's claims
-            if (LOCAL)
-                old[k] = __hip_atomic_fetch_or(s.visited + (u[k] >> 5), bit, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-            else
-                old[k] = __hip_atomic_fetch_or(s.visited + (u[k] >> 5), bit, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
-    // (the empty asm keeps the compiler from folding a use into the
-    // atomic's own block)
-#pragma unroll
-    for (int k = 0; k < kFastK; k++) asm volatile("" : "+v"(old[k]));
-    unsigned fresh = 0;
-#pragma unroll
-    for (int k = 0; k < kFastK; k++)
-        if (!((old[k] >> (u[k] & 31)) & 1u)) fresh |= 1u << k;
-    if (!EXACT) {
-        // best-effort culling elsewhere may have lost bits of labelled vertices
-        int dep[kFastK];
-#pragma unroll
-        for (int k = 0; k < kFastK; k++) dep[k] = ((fresh >> k) & 1) ? s.depth[u[k]] : -1;
-#pragma unroll
-        for (int k = 0; k < kFastK; k++)
-            if (dep[k] != -1) fresh &= ~(1u << k);
-    }
-#pragma unroll
-    for (int k = 0; k < kFastK; k++) {
-        ub[k] = 0;
-        ud[k] = 0;
+    for (int k = 0; k < S; k++) {
         if ((fresh >> k) & 1) {
-            s.depth[u[k]] = next_depth;
-            if (PRED) s.pred[u[k]] = pv[k];
-            ub[k] = g.row[u[k]];
-            ud[k] = g.row[u[k] + 1];
+            const int d = ud[k] - ub[k];
+            fe += (unsigned) d;
+            dmax = max(dmax, d);
+            labeled = true;
+            if (p < (unsigned) kFastCap) {
+                F.v[dst][p] = u[k];
+                F.beg[dst][p] = ub[k];
+                F.deg[dst][p] = d;
+            } else if (p < s.cap) {
+                spill[p] = u[k];
+            }
+            p++;
         }
     }
-    return fresh;
 }
 
 // One level from LDS buffer src (n members, 0 < n <= kFastCap) into buffer
 // src ^ 1; members past kFastCap go to the global queue of step c + 1 at the
-// same positions.  Every thread calls it.
+// same positions.  When the members' rows fit kFastDirect slots per thread
+// (every lattice level), thread t takes members t and t + kBlock whole;
+// otherwise the level's edges are spread evenly over the threads through
+// the degree scan, kFastK per thread per batch.  Every thread calls it.
 template <bool PRED, bool EXACT>
 __device__ __forceinline__ void fast_lds_level(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, int src, FastFront &F)
 {
@@ -1399,58 +1455,61 @@ __device__ __forceinline__ void fast_lds_level(const GraphArgs &g, const SearchA
     const int lane = lane_id(), wave = threadIdx.x / kWave;
     const int next_depth = c.iteration + 1;
     int *spill = s.q[(c.step + 1) & 1];
-    if (threadIdx.x == 0) F.count = 0;  // ordered before the appends by the scan's barriers
-    const int total = fast_scan(F, src, n);
+    const int maxd = F.maxdeg[src];
+    const int per = ((int) n + kBlock - 1) / kBlock;  // members per thread: 1 or 2
+    if (threadIdx.x == 0) F.count = 0;
     unsigned long long fe = 0;
     bool labeled = false;
-    for (int b0 = 0; b0 < total; b0 += kBlock * kFastK) {
-        int u[kFastK], pv[kFastK], ub[kFastK], ud[kFastK];
-        const unsigned fresh = fast_edges<PRED, EXACT, true>(g, s, next_depth, F.deg[src], F.beg[src], F.v[src], n, total,
-                                                       b0 + (int) threadIdx.x * kFastK, u, pv, ub, ud);
-        // append the new members: one LDS reservation per wave
-        const int cnt = __popc(fresh);
-        const int incl = wave_inclusive_sum(cnt);
-        const int wtot = readlane(incl, kWave - 1);
-        if (wtot) {
-            unsigned base = 0;
-            if (lane == 0) base = atomicAdd(&F.count, (unsigned) wtot);
-            unsigned p = (unsigned) readlane((int) base, 0) + (unsigned) (incl - cnt);
+    int dmax = 0;
+    if (maxd * per <= kFastDirect) {
+        __syncthreads();  // the resets above precede every append
+        int at[kFastDirect], pv[kFastDirect];
 #pragma unroll
-            for (int k = 0; k < kFastK; k++) {
-                if ((fresh >> k) & 1) {
-                    const int d = ud[k] - ub[k];
-                    fe += (unsigned) d;
-                    labeled = true;
-                    if (p < (unsigned) kFastCap) {
-                        F.v[dst][p] = u[k];
-                        F.beg[dst][p] = ub[k];
-                        F.deg[dst][p] = d;
-                    } else if (p < s.cap) {
-                        spill[p] = u[k];
-                    }
-                    p++;
-                }
+        for (int j = 0; j < kFastDirect; j++) {
+            const int m = per == 1 ? 0 : j / (kFastDirect / 2);
+            const int k = per == 1 ? j : j % (kFastDirect / 2);
+            const int i = (int) threadIdx.x + m * kBlock;
+            at[j] = -1;
+            pv[j] = -1;
+            if (i < (int) n && k < F.deg[src][i]) {
+                at[j] = F.beg[src][i] + k;
+                pv[j] = F.v[src][i];
             }
         }
+        int u[kFastDirect], ub[kFastDirect], ud[kFastDirect];
+        const unsigned fresh = fast_claim_slots<PRED, EXACT, true, kFastDirect>(g, s, next_depth, at, pv, u, ub, ud);
+        fast_append<kFastDirect>(s, F, dst, spill, fresh, u, ub, ud, fe, labeled, dmax);
+    } else {
+        const int total = fast_scan(F, src, n);  // (its barriers order the resets before the appends)
+        for (int b0 = 0; b0 < total; b0 += kBlock * kFastK) {
+            int u[kFastK], pv[kFastK], ub[kFastK], ud[kFastK];
+            const unsigned fresh = fast_edges<PRED, EXACT, true>(g, s, next_depth, F.deg[src], F.beg[src], F.v[src], n,
+                                                                 total, b0 + (int) threadIdx.x * kFastK, u, pv, ub, ud);
+            fast_append<kFastK>(s, F, dst, spill, fresh, u, ub, ud, fe, labeled, dmax);
+        }
     }
-    // fold the threads' edge sums and label flags
+    // fold the threads' edge sums, label flags and largest degrees
     const int lo = wave_inclusive_sum((int) (fe & 0xFFFFu));
     const int hi = wave_inclusive_sum((int) (fe >> 16));
+    const int wmax = wave_inclusive_max_nonneg(dmax);
     const bool any = ballot(labeled) != 0;
     if (lane == kWave - 1) {
         F.wfe[wave] = ((unsigned long long) (unsigned) hi << 16) + (unsigned long long) (unsigned) lo;
         F.wlab[wave] = any ? 1 : 0;
+        F.wmax[wave] = wmax;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long f = 0;
-        int l = 0;
+        int l = 0, dm = 0;
         for (int w = 0; w < kWaves; w++) {
             f += F.wfe[w];
             l |= F.wlab[w];
+            dm = max(dm, F.wmax[w]);
         }
         F.fe = f;
         F.labeled = l;
+        F.maxdeg[dst] = dm;
     }
     __syncthreads();
 }
@@ -1560,12 +1619,23 @@ __device__ __forceinline__ void fast_lds_path(const GraphArgs &g, const SearchAr
                               unsigned long long fv, FastFront &F, bool wait_others)
 {
     const int *in = s.q[c.step & 1];
+    int dmax = 0;
     for (unsigned i = threadIdx.x; i < n; i += kBlock) {
         const int v = ld_queue(in + i);
         const int b = g.row[v];
+        const int d = g.row[v + 1] - b;
         F.v[0][i] = v;
         F.beg[0][i] = b;
-        F.deg[0][i] = g.row[v + 1] - b;
+        F.deg[0][i] = d;
+        dmax = max(dmax, d);
+    }
+    dmax = wave_inclusive_max_nonneg(dmax);
+    if (lane_id() == kWave - 1) F.wmax[threadIdx.x / kWave] = dmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int dm = 0;
+        for (int w = 0; w < kWaves; w++) dm = max(dm, F.wmax[w]);
+        F.maxdeg[0] = dm;
     }
     __syncthreads();
     int src = 0;
