@@ -25,7 +25,10 @@ def batched_rmat(num_graphs=64, num_queries=256, scale_range=(14, 20), ef_range=
     return specs, graphs, gids, srcs
 
 
-def serving_pool(er_log_sizes=range(16, 24), rmat_scale=24, rmat_ef=16, seed=0x5E4E):
+SERVING_SEED = 0x5E4E
+
+
+def serving_pool(er_log_sizes=range(16, 24), rmat_scale=24, rmat_ef=16, seed=SERVING_SEED):
     """Returns (names, graphs)."""
     names, graphs = [], []
     for k in er_log_sizes:
