@@ -466,7 +466,13 @@ def main():
             if k in tr:
                 measured[k] = tr[k]
         if "phases" in traffic:
-            measured["phases"] = traffic["phases"]
+            # per-level launches of the two-phase strategy: each phase's
+            # counter bandwidth (profiles/step_kernels_rmat22_tp_iterative.md)
+            measured["phases"] = {"workload": "two-phase per-level launches",
+                                  "profiled_at": traffic["phases"].get("commit")}
+            for ph, v in traffic["phases"]["phases"].items():
+                measured["phases"][ph] = {"avg_us": v["avg_us"], "counter_gbs": v["counter_gbs"],
+                                          "counter_frac": v["counter_frac"]}
 
     # Top-down expand-contract over the same sources, for reference.
     top_down = None
