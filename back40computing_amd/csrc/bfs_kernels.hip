@@ -1461,7 +1461,7 @@ __device__ __forceinline__ void fast_lds_level(const GraphArgs &g, const SearchA
     unsigned long long fe = 0;
     bool labeled = false;
     int dmax = 0;
-    if (maxd * per <= kFastDirect) {
+    if (maxd <= kFastDirect / per) {  // (a product could overflow for huge rows)
         __syncthreads();  // the resets above precede every append
         int at[kFastDirect], pv[kFastDirect];
 #pragma unroll
