@@ -16,7 +16,6 @@
 namespace b40c_hip {
 
 constexpr int kWave = 64;
-constexpr int kWaveQueueCap = 512;             // LDS staging entries per wave
 constexpr int kHistory = 1024;                 // LDS history-hash entries per workgroup
 constexpr int kBarrierFanIn = 16;              // first-level barrier counters
 constexpr int kHubChunk = 2048;                // edges per grid-tier chunk
@@ -99,6 +98,15 @@ inline namespace B40C_NS_CAT(block, B40C_BLOCK) {
 constexpr int kBlock = B40C_BLOCK;
 constexpr int kWaves = kBlock / kWave;         // waves per workgroup
 static_assert(kBlock % kWave == 0 && kBlock >= 128 && kBlock <= 1024, "workgroup of whole waves");
+
+// LDS staging entries per wave of the frontier queues.  Every full area
+// costs one reservation atomic on the queue's length, and on a large
+// level those atomics, all on one address, bound the step: 2048 entries
+// per wave took the single-source two-phase search on R-MAT s22 from 5.09
+// to 2.64 ms (64 KiB of LDS per 512-thread workgroup, 128 KiB with the
+// predecessor staging).  The batch kernels keep 512, which leaves them
+// their occupancy of three 256-thread workgroups per CU.
+constexpr int kWaveQueueCap = kBlock >= 512 ? 2048 : 512;
 
 // ---------------------------------------------------------------------------
 // Wave64 primitives
