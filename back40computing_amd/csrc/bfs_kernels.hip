@@ -974,7 +974,7 @@ __device__ __forceinline__ void body_expand_mark(const GraphArgs &g, const Searc
                 int u[4];
                 unsigned w[4];
 #pragma unroll
-                for (int j = 0; j < 4; j++) u[j] = at[j] >= 0 ? __builtin_nontemporal_load(g.col + at[j]) : -1;
+                for (int j = 0; j < 4; j++) u[j] = at[j] >= 0 ? g.col[at[j]] : -1;
 #pragma unroll
                 for (int j = 0; j < 4; j++) w[j] = u[j] >= 0 ? s.visited[u[j] >> 5] : ~0u;
                 bool todo[4];
@@ -1009,7 +1009,7 @@ __device__ __forceinline__ void body_expand_mark(const GraphArgs &g, const Searc
 #pragma unroll
             for (int j = 0; j < U; j++) {
                 const int idx = k + j * kBlock + (int) threadIdx.x;
-                u[j] = idx < e ? __builtin_nontemporal_load(g.col + idx) : -1;
+                u[j] = idx < e ? g.col[idx] : -1;
             }
 #pragma unroll
             for (int j = 0; j < U; j++) w[j] = u[j] >= 0 ? s.visited[u[j] >> 5] : ~0u;
