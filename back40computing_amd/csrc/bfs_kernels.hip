@@ -361,17 +361,17 @@ __device__ __forceinline__ void body_expand_contract(const GraphArgs &g, const S
 //      (prefetched one step ahead) give the vertices still to search with a
 //      non-empty row; they are queued in the wave's LDS -- on later levels
 //      only a few lanes of a group have work;
-//   2. every 64 queued vertices are scanned with all lanes busy: four
-//      neighbours per lane per round, all loads of a round in flight; rows
-//      of kBuLong+ neighbours get one such round and then a wave-wide scan
-//      of the rest (a high-degree vertex usually finds a frontier neighbour
-//      among its first four).
+//   2. every 128 queued vertices are scanned with all lanes busy, two per
+//      lane: four neighbours of each per round, all eight loads of a round
+//      in flight; rows of kBuLong+ neighbours get one such round and then a
+//      wave-wide scan of the rest (a high-degree vertex usually finds a
+//      frontier neighbour among its first four).
 // Parents found are collected as bits in LDS per window of kBuWindow groups
 // and written once per word: visited |= found, next frontier = found (the
 // next-frontier word is overwritten in full, which also clears it).
 constexpr int kBuLong = 64;
 constexpr int kBuGroups = 2;
-constexpr int kBuQueue = 128;   // queued vertices per wave (two scans' worth)
+constexpr int kBuQueue = 192;   // queued vertices per wave (a two-entry scan and a group more)
 constexpr int kBuWindow = 32;   // groups per window of found bits (2 words each)
 static_assert(3 * kBuQueue + 2 * kBuWindow <= kWaveQueueCap, "bottom-up LDS use fits the wave's staging area");
 
@@ -407,55 +407,77 @@ __device__ __forceinline__ void body_bottom_up(const GraphArgs &g, const SearchA
     const unsigned g_begin = gw * per;
     const unsigned g_end = min(groups, (gw + 1) * per);
 
-    // Scans the first cnt (1..64, wave-uniform) queued vertices of the
-    // window whose first visited word is word0.
+    // Scans the first cnt (1..128, wave-uniform) queued vertices of the
+    // window whose first visited word is word0: lane l takes entries l and
+    // l + 64, whose rounds of four neighbours run with all loads in flight.
     auto scan = [&](int cnt, unsigned word0) {
-        int v = -1, beg = 0, end = 0;
-        if (lane < cnt) {
-            v = qv[lane];
-            beg = qb[lane];
-            end = qe[lane];
-        }
-        int pos = beg;
-        const int lim = end - beg < kBuLong ? end : min(end, beg + 4);
-        int parent = -1;
-        while (parent < 0 && pos < lim) {
-            int u[4];
-            unsigned fw[4];
+        int v[2], beg[2], end[2], pos[2], lim[2], parent[2];
 #pragma unroll
-            for (int j = 0; j < 4; j++) u[j] = pos + j < lim ? g.rcol[pos + j] : -1;
-#pragma unroll
-            for (int j = 0; j < 4; j++) fw[j] = u[j] >= 0 ? cur[u[j] >> 5] : 0u;
-#pragma unroll
-            for (int j = 3; j >= 0; j--)  // the first hit in row order wins
-                if (u[j] >= 0 && ((fw[j] >> (u[j] & 31)) & 1u)) parent = u[j];
-            pos += 4;
-        }
-        uint64_t lm = ballot(parent < 0 && lim < end);
-        while (lm) {
-            const int l = uniform(__builtin_ctzll(lm));
-            lm &= lm - 1;
-            const int b = readlane(lim, l), e = readlane(end, l);
-            int hit = -1;
-            for (int q = b; q < e; q += kWave) {
-                const int idx = q + lane;
-                const int uu = idx < e ? g.rcol[idx] : -1;
-                const uint64_t hm = ballot(uu >= 0 && ((cur[uu >> 5] >> (uu & 31)) & 1u));
-                if (hm) {
-                    hit = readlane(uu, uniform(__builtin_ctzll(hm)));
-                    break;
-                }
+        for (int h = 0; h < 2; h++) {
+            const int i = lane + h * kWave;
+            v[h] = -1;
+            beg[h] = 0;
+            end[h] = 0;
+            if (i < cnt) {
+                v[h] = qv[i];
+                beg[h] = qb[i];
+                end[h] = qe[i];
             }
-            if (lane == l) parent = hit;
+            pos[h] = beg[h];
+            lim[h] = end[h] - beg[h] < kBuLong ? end[h] : min(end[h], beg[h] + 4);
+            parent[h] = -1;
         }
-        if (parent >= 0) {
-            s.depth[v] = next_depth;
-            if (PRED) s.pred[v] = parent;
-            fe += (unsigned) (end - beg);
-            labeled = true;
-            found++;
-            __hip_atomic_fetch_or(&fbits[((unsigned) v >> 5) - word0], 1u << (v & 31), __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_WAVEFRONT);
+        while ((parent[0] < 0 && pos[0] < lim[0]) || (parent[1] < 0 && pos[1] < lim[1])) {
+            int u[8];
+            unsigned fw[8];
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    u[h * 4 + j] = parent[h] < 0 && pos[h] + j < lim[h] ? g.rcol[pos[h] + j] : -1;
+#pragma unroll
+            for (int k = 0; k < 8; k++) fw[k] = u[k] >= 0 ? cur[u[k] >> 5] : 0u;
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int j = 3; j >= 0; j--)  // the first hit in row order wins
+                    if (u[h * 4 + j] >= 0 && ((fw[h * 4 + j] >> (u[h * 4 + j] & 31)) & 1u)) parent[h] = u[h * 4 + j];
+            pos[0] += 4;
+            pos[1] += 4;
+        }
+        // rows of kBuLong+ neighbours with no parent among their first four:
+        // a wave-wide scan of the rest, one row at a time
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            uint64_t lm = ballot(parent[h] < 0 && lim[h] < end[h]);
+            while (lm) {
+                const int l = uniform(__builtin_ctzll(lm));
+                lm &= lm - 1;
+                const int b = readlane(lim[h], l), e = readlane(end[h], l);
+                int hit = -1;
+                for (int q = b; q < e; q += kWave) {
+                    const int idx = q + lane;
+                    const int uu = idx < e ? g.rcol[idx] : -1;
+                    const uint64_t hm = ballot(uu >= 0 && ((cur[uu >> 5] >> (uu & 31)) & 1u));
+                    if (hm) {
+                        hit = readlane(uu, uniform(__builtin_ctzll(hm)));
+                        break;
+                    }
+                }
+                if (lane == l) parent[h] = hit;
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            if (parent[h] >= 0) {
+                s.depth[v[h]] = next_depth;
+                if (PRED) s.pred[v[h]] = parent[h];
+                fe += (unsigned) (end[h] - beg[h]);
+                labeled = true;
+                found++;
+                __hip_atomic_fetch_or(&fbits[((unsigned) v[h] >> 5) - word0], 1u << (v[h] & 31), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WAVEFRONT);
+            }
         }
     };
     // Moves queue entries [from, from + n) to the front.
@@ -532,17 +554,17 @@ __device__ __forceinline__ void body_bottom_up(const GraphArgs &g, const SearchA
                     qe[p] = end[k];
                 }
                 qn += __popcll(m);
-                if (qn >= kWave) {
+                if (qn >= 2 * kWave) {
                     wave_lds_sync();
-                    scan(kWave, word0);
-                    shift(kWave, qn - kWave);
-                    qn -= kWave;
+                    scan(2 * kWave, word0);
+                    shift(2 * kWave, qn - 2 * kWave);
+                    qn -= 2 * kWave;
                 }
             }
         }
         wave_lds_sync();
         while (qn > 0) {
-            const int cnt = min(qn, kWave);
+            const int cnt = min(qn, 2 * kWave);
             scan(cnt, word0);
             if (qn > cnt) shift(cnt, qn - cnt);
             qn -= cnt;
