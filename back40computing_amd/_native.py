@@ -44,6 +44,11 @@ class DeviceCsrStruct(ctypes.Structure):
     ]
 
 
+# BFS_FAST_VISITED_MAX (include/b40c_bfs.h): most vertices for which the fast
+# path keeps its copy of the visited bitmap in LDS (config fast_visited = 1).
+FAST_VISITED_MAX = 1103872
+
+
 class BfsConfig(ctypes.Structure):
     _fields_ = [
         ("wave_gather_threshold", ctypes.c_int32),
@@ -62,6 +67,7 @@ class BfsConfig(ctypes.Structure):
         ("claim_filter", ctypes.c_int32),
         ("spread_steps", ctypes.c_int32),
         ("compaction", ctypes.c_int32),
+        ("fast_visited", ctypes.c_int32),
     ]
 
 

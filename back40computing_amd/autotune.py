@@ -58,6 +58,7 @@ SPACE_LATTICE = {
     "wave_gather_threshold": [8, 32, 64],
     "cta_gather_threshold": [256, 512, 1024],
     "compaction": [1, 2],
+    "fast_visited": [1, 2],
 }
 
 SPACE_BATCH = {

@@ -63,7 +63,8 @@ const bfs_config kDefaults = {
     /*grab_strided*/ 1,
     /*claim_filter*/ 1,
     /*spread_steps*/ 1,
-    /*compaction*/ 1};
+    /*compaction*/ 1,
+    /*fast_visited*/ 1};
 
 constexpr unsigned kHubCapacity = 1u << 16;  // grid-tier entries per step list
 
@@ -536,6 +537,8 @@ bfs_status bfs_problem_set_config(bfs_problem *p, const bfs_config *cfg)
     if (c.spread_steps == 0) c.spread_steps = kDefaults.spread_steps;
     if (c.compaction == 0) c.compaction = kDefaults.compaction;
     if (c.compaction < 1 || c.compaction > 2) return fail(BFS_ERR_INVALID_ARG, "unknown compaction mode");
+    if (c.fast_visited == 0) c.fast_visited = kDefaults.fast_visited;
+    if (c.fast_visited < 1 || c.fast_visited > 2) return fail(BFS_ERR_INVALID_ARG, "unknown fast_visited mode");
     if (c.do_alpha == 0) c.do_alpha = kDefaults.do_alpha;
     if (c.do_beta == 0) c.do_beta = kDefaults.do_beta;
     if (c.do_alpha < 1 || c.do_beta < 1) return fail(BFS_ERR_INVALID_ARG, "invalid direction switch points");
@@ -895,6 +898,7 @@ void apply_search_config(SearchArgs &s, const bfs_config &cfg, int grid)
     s.grab_max = cfg.tile_grab_max > 0 ? cfg.tile_grab_max : 1;
     s.coherent_filter = cfg.claim_filter == 2;
     s.spread = cfg.spread_steps == 1;
+    s.fast_visited = cfg.fast_visited == 1;
     s.fast_t = cfg.fast_path_threshold;
     // the reference leaves at grid * 512 items for 128-thread CTAs
     // (enactor_hybrid.cuh:571): four items per thread of the grid

@@ -54,6 +54,7 @@ struct SearchArgs {
     int grab_max;          // most tiles taken per tile-counter atomic
     int coherent_filter;   // claim pre-test reads the device-coherent visited word
     int spread;            // few-member top-down levels split their edges evenly over the grid
+    int fast_visited;      // the fast path claims in an LDS copy of the visited bitmap (when it fits)
     // Chained compaction of contract and filter steps: one status word per
     // tile, tagged with epoch scan_epoch + step (never 0: cleared words read
     // as unpublished).  Null tile_status: wave-staged atomic reservation.

@@ -1262,9 +1262,9 @@ constexpr int kFastK = 4;       // consecutive edges per thread per batch
 constexpr unsigned long long kFastEdges = 4ull * kBlock * kFastK;
 static_assert(kFastCap <= 4 * kBlock, "the degree scan takes four members per thread");
 
-// Direct levels: when every member's degree is small, thread t takes the
-// whole rows of members t and t + kBlock (up to kFastDirect edges) with no
-// degree scan and no search for each edge's member.
+// Direct levels: when every member's degree is small, groups of lanes take
+// whole rows (a thread up to kFastDirect edges) with no degree scan and no
+// search for each edge's member.
 constexpr int kFastDirect = 8;
 
 struct FastFront {
@@ -1281,13 +1281,31 @@ struct FastFront {
     int wmax[kWaves];
 };
 
+// FASTV: the fast path's copy of the visited bitmap takes the rest of the
+// workgroup's LDS (gfx950: 160 KiB) once the frontier buffers and 1 KiB for
+// the level loop's other variables are placed: graphs of up to
+// kFastVisitedWords * 32 vertices.  Only the expand-contract and
+// direction-optimizing loops have an instantiation with it; the others (and
+// larger graphs) keep the smaller LDS footprint.
+constexpr int kLdsBytes = 163840;
+constexpr int kFastVisitedWords = ((kLdsBytes - 1024 - (int) sizeof(FastFront)) / 4) & ~63;
+static_assert(kFastVisitedWords * 32 == BFS_FAST_VISITED_MAX, "the header states the fast path's LDS bitmap size");
+
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+template <bool FASTV>
+struct FastLds {
+    FastFront f;
+    __attribute__((aligned(16))) unsigned vbits[FASTV ? kFastVisitedWords : 4];
+};
+
 // The level loop's LDS: the step bodies' staging and the fast path's
 // frontier are never live at once (the fast path starts after a grid
 // barrier and re-initialises the step state when it ends).
-template <bool PRED>
+template <bool PRED, bool FASTV>
 union LoopLds {
     Lds<PRED> step;
-    FastFront fast;
+    FastLds<FASTV> fast;
 };
 
 // Exclusive scan of F.deg[b][0, n) in place; returns the sum to every thread.
@@ -1323,42 +1341,56 @@ __device__ int fast_scan(FastFront &F, int b, unsigned n)
 // agent-scope ones must be coherent across the eight XCDs; the barrier
 // before the path invalidated stale lines, and the one after it writes the
 // path's lines back.  Column entries are read through the caches: the next
-// level's rows sit next to this level's on a lattice.
+// level's rows sit next to this level's on a lattice.  With vbits (LOCAL
+// only) the claims are LDS atomics on the path's copy of the visited
+// bitmap: no pre-read and no memory trip between a column entry and the
+// claimed vertex's row bounds.
 template <bool PRED, bool EXACT, bool LOCAL, int S>
 __device__ __forceinline__ unsigned fast_claim_slots(const GraphArgs &g, const SearchArgs &s, int next_depth,
                                                      const int (&at)[S], const int (&pv)[S], int (&u)[S],
-                                                     int (&ub)[S], int (&ud)[S])
+                                                     int (&ub)[S], int (&ud)[S], bool ldsv = false,
+                                                     unsigned *vbits = nullptr)
 {
 #pragma unroll
     for (int k = 0; k < S; k++)
         u[k] = at[k] < 0 ? -1 : LOCAL ? g.col[at[k]] : __builtin_nontemporal_load(g.col + at[k]);
-    // visited words (a set bit is final), then claims of the clear ones
-    unsigned cur[S];
-#pragma unroll
-    for (int k = 0; k < S; k++) {
-        cur[k] = ~0u;
-        if (u[k] >= 0) {
-            unsigned *w = s.visited + (u[k] >> 5);
-            cur[k] = s.coherent_filter ? __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *w;
-        }
-    }
-    // The claims' results are read only after all are issued (a use
-    // next to its atomic would wait out each round trip in turn).
     unsigned old[S];
+    if (LOCAL && ldsv) {
 #pragma unroll
-    for (int k = 0; k < S; k++) {
-        old[k] = ~0u;
-        const unsigned bit = 1u << (u[k] & 31);
-        if (u[k] >= 0 && !(cur[k] & bit)) {
-            // agent scope unless LOCAL: other workgroups claim concurrently,
-            // and another XCD's L2 may hold this word from before an
-            // earlier step's claims
-            if (LOCAL)
-                old[k] = __hip_atomic_fetch_or(s.visited + (u[k] >> 5), bit, __ATOMIC_RELAXED,
+        for (int k = 0; k < S; k++) {
+            old[k] = ~0u;
+            if (u[k] >= 0)
+                old[k] = __hip_atomic_fetch_or(vbits + (u[k] >> 5), 1u << (u[k] & 31), __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_WORKGROUP);
-            else
-                old[k] = __hip_atomic_fetch_or(s.visited + (u[k] >> 5), bit, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+    } else {
+        // visited words (a set bit is final), then claims of the clear ones
+        unsigned cur[S];
+#pragma unroll
+        for (int k = 0; k < S; k++) {
+            cur[k] = ~0u;
+            if (u[k] >= 0) {
+                unsigned *w = s.visited + (u[k] >> 5);
+                cur[k] = s.coherent_filter ? __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *w;
+            }
+        }
+        // The claims' results are read only after all are issued (a use
+        // next to its atomic would wait out each round trip in turn).
+#pragma unroll
+        for (int k = 0; k < S; k++) {
+            old[k] = ~0u;
+            const unsigned bit = 1u << (u[k] & 31);
+            if (u[k] >= 0 && !(cur[k] & bit)) {
+                // agent scope unless LOCAL: other workgroups claim
+                // concurrently, and another XCD's L2 may hold this word from
+                // before an earlier step's claims
+                if (LOCAL)
+                    old[k] = __hip_atomic_fetch_or(s.visited + (u[k] >> 5), bit, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                else
+                    old[k] = __hip_atomic_fetch_or(s.visited + (u[k] >> 5), bit, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
     }
     // (the empty asm keeps the compiler from folding a use into the
@@ -1400,7 +1432,8 @@ template <bool PRED, bool EXACT, bool LOCAL>
 __device__ __forceinline__ unsigned fast_edges(const GraphArgs &g, const SearchArgs &s, int next_depth,
                                                const int *offs, const int *begs, const int *vs, unsigned n,
                                                int total, int e0, int (&u)[kFastK], int (&pv)[kFastK],
-                                               int (&ub)[kFastK], int (&ud)[kFastK])
+                                               int (&ub)[kFastK], int (&ud)[kFastK], bool ldsv = false,
+                                               unsigned *vbits = nullptr)
 {
     // member holding edge e0: the last i with offs[i] <= e0 (it has edges)
     int i = 0;
@@ -1427,7 +1460,7 @@ __device__ __forceinline__ unsigned fast_edges(const GraphArgs &g, const SearchA
             pv[k] = vs[i];
         }
     }
-    return fast_claim_slots<PRED, EXACT, LOCAL, kFastK>(g, s, next_depth, at, pv, u, ub, ud);
+    return fast_claim_slots<PRED, EXACT, LOCAL, kFastK>(g, s, next_depth, at, pv, u, ub, ud, ldsv, vbits);
 }
 
 // Appends one thread's claimed slots to LDS buffer dst (one LDS reservation
@@ -1467,30 +1500,37 @@ __device__ __forceinline__ void fast_append(const SearchArgs &s, FastFront &F, i
 // One level from LDS buffer src (n members, 0 < n <= kFastCap) into buffer
 // src ^ 1; members past kFastCap go to the global queue of step c + 1 at the
 // same positions.  When the members' rows fit kFastDirect slots per thread
-// (every lattice level), thread t takes members t and t + kBlock whole;
-// otherwise the level's edges are spread evenly over the threads through
-// the degree scan, kFastK per thread per batch.  Every thread calls it.
+// (every lattice level), each member's row goes to a group of consecutive
+// lanes, one edge per lane; otherwise the level's edges are spread evenly
+// over the threads through the degree scan, kFastK per thread per batch.
+// Every thread calls it.
 template <bool PRED, bool EXACT>
-__device__ __forceinline__ void fast_lds_level(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, int src, FastFront &F)
+__device__ __forceinline__ void fast_lds_level(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, int src,
+                                               FastFront &F, bool ldsv, unsigned *vbits)
 {
     const int dst = src ^ 1;
     const int lane = lane_id(), wave = threadIdx.x / kWave;
     const int next_depth = c.iteration + 1;
     int *spill = s.q[(c.step + 1) & 1];
     const int maxd = F.maxdeg[src];
-    const int per = ((int) n + kBlock - 1) / kBlock;  // members per thread: 1 or 2
     if (threadIdx.x == 0) F.count = 0;
     unsigned long long fe = 0;
     bool labeled = false;
     int dmax = 0;
-    if (maxd <= kFastDirect / per) {  // (a product could overflow for huge rows)
+    // Direct level: 2^lg consecutive lanes per member (2^lg >= its largest
+    // degree), lane k of a group taking edge k, so that the column entries
+    // one load instruction reads are whole rows side by side; kFastDirect
+    // such passes cover up to kBlock * kFastDirect / 2^lg members.
+    const int lg = maxd <= 1 ? 0 : maxd <= 2 ? 1 : maxd <= 4 ? 2 : maxd <= kFastDirect ? 3 : 30;
+    static_assert(kFastDirect == 8, "lg spans one to eight lanes per member");
+    if (lg < 30 && ((int) n << lg) <= kBlock * kFastDirect) {
         __syncthreads();  // the resets above precede every append
         int at[kFastDirect], pv[kFastDirect];
+        const int k = (int) threadIdx.x & ((1 << lg) - 1);
+        const int i0 = (int) threadIdx.x >> lg;
 #pragma unroll
         for (int j = 0; j < kFastDirect; j++) {
-            const int m = per == 1 ? 0 : j / (kFastDirect / 2);
-            const int k = per == 1 ? j : j % (kFastDirect / 2);
-            const int i = (int) threadIdx.x + m * kBlock;
+            const int i = i0 + ((j * kBlock) >> lg);
             at[j] = -1;
             pv[j] = -1;
             if (i < (int) n && k < F.deg[src][i]) {
@@ -1499,14 +1539,16 @@ __device__ __forceinline__ void fast_lds_level(const GraphArgs &g, const SearchA
             }
         }
         int u[kFastDirect], ub[kFastDirect], ud[kFastDirect];
-        const unsigned fresh = fast_claim_slots<PRED, EXACT, true, kFastDirect>(g, s, next_depth, at, pv, u, ub, ud);
+        const unsigned fresh =
+            fast_claim_slots<PRED, EXACT, true, kFastDirect>(g, s, next_depth, at, pv, u, ub, ud, ldsv, vbits);
         fast_append<kFastDirect>(s, F, dst, spill, fresh, u, ub, ud, fe, labeled, dmax);
     } else {
         const int total = fast_scan(F, src, n);  // (its barriers order the resets before the appends)
         for (int b0 = 0; b0 < total; b0 += kBlock * kFastK) {
             int u[kFastK], pv[kFastK], ub[kFastK], ud[kFastK];
-            const unsigned fresh = fast_edges<PRED, EXACT, true>(g, s, next_depth, F.deg[src], F.beg[src], F.v[src], n,
-                                                                 total, b0 + (int) threadIdx.x * kFastK, u, pv, ub, ud);
+            const unsigned fresh =
+                fast_edges<PRED, EXACT, true>(g, s, next_depth, F.deg[src], F.beg[src], F.v[src], n, total,
+                                              b0 + (int) threadIdx.x * kFastK, u, pv, ub, ud, ldsv, vbits);
             fast_append<kFastK>(s, F, dst, spill, fresh, u, ub, ud, fe, labeled, dmax);
         }
     }
@@ -1636,11 +1678,38 @@ __device__ __forceinline__ void spread_step(const GraphArgs &g, const SearchArgs
 // counters and the fast_* hand-off exactly as the ordinary steps would have.
 // `fv`: visited-edge sum through step c (see choose_direction).  Block 0
 // only; every thread calls it.
-template <bool PRED, bool EXACT>
+template <bool PRED, bool EXACT, bool FASTV>
 __device__ __forceinline__ void fast_lds_path(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, int loop,
-                              unsigned long long fv, FastFront &F, bool wait_others)
+                              unsigned long long fv, FastLds<FASTV> &L, bool wait_others)
 {
+    FastFront &F = L.f;
     const int *in = s.q[c.step & 1];
+    // FASTV: the path's claims go to an LDS copy of the visited bitmap (no
+    // other workgroup reads or writes visited words until the grid barrier
+    // that ends the path); it is written back before that barrier.
+    const bool ldsv = FASTV && s.nodes <= kFastVisitedWords * 32;
+    const int vwords = (s.nodes + 31) >> 5;
+    if (ldsv) {
+        // 16-byte chunks, kLoad per thread in flight together
+        constexpr int kLoad = 16;
+        const u32x4 *from = reinterpret_cast<const u32x4 *>(s.visited);
+        u32x4 *to = reinterpret_cast<u32x4 *>(L.vbits);
+        const int chunks = vwords >> 2;
+        for (int base = 0; base < chunks; base += kBlock * kLoad) {
+            u32x4 t[kLoad];
+#pragma unroll
+            for (int j = 0; j < kLoad; j++) {
+                const int i = base + j * kBlock + (int) threadIdx.x;
+                if (i < chunks) t[j] = from[i];
+            }
+#pragma unroll
+            for (int j = 0; j < kLoad; j++) {
+                const int i = base + j * kBlock + (int) threadIdx.x;
+                if (i < chunks) to[i] = t[j];
+            }
+        }
+        for (int i = 4 * chunks + (int) threadIdx.x; i < vwords; i += kBlock) L.vbits[i] = s.visited[i];
+    }
     int dmax = 0;
     for (unsigned i = threadIdx.x; i < n; i += kBlock) {
         const int v = ld_queue(in + i);
@@ -1671,7 +1740,7 @@ __device__ __forceinline__ void fast_lds_path(const GraphArgs &g, const SearchAr
             s.ctl->level_clock[c.step] = wall_clock64();
         }
         queued += n;
-        fast_lds_level<PRED, EXACT>(g, s, c, n, src, F);
+        fast_lds_level<PRED, EXACT>(g, s, c, n, src, F, ldsv, L.vbits);
         n2 = F.count;
         fe2 = F.fe;
         if (F.labeled) maxd = c.iteration + 1;
@@ -1687,6 +1756,14 @@ __device__ __forceinline__ void fast_lds_path(const GraphArgs &g, const SearchAr
         n = n2;
         if (!go) break;
         __syncthreads();  // every thread has read F's results before the next level resets them
+    }
+    if (ldsv) {
+        // (the last level's claims precede its closing workgroup barrier)
+        const u32x4 *from = reinterpret_cast<const u32x4 *>(L.vbits);
+        u32x4 *to = reinterpret_cast<u32x4 *>(s.visited);
+        const int chunks = vwords >> 2;
+        for (int i = threadIdx.x; i < chunks; i += kBlock) to[i] = from[i];
+        for (int i = 4 * chunks + (int) threadIdx.x; i < vwords; i += kBlock) s.visited[i] = L.vbits[i];
     }
     // The hand-off below rewrites counters the other workgroups may still
     // be reading at the top of the step the path started from, unless they
@@ -1804,11 +1881,11 @@ __device__ __forceinline__ void publish_status(const SearchArgs &s)
 // ---------------------------------------------------------------------------
 // One instantiation per strategy (LOOP), so profiles attribute time to the
 // strategy and the level loop carries no dead strategy branches.
-template <bool PRED, bool EXACT, int LOOP>
+template <bool PRED, bool EXACT, int LOOP, bool FASTV>
 __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchArgs s, Cursor c)
 {
     constexpr int loop = LOOP;
-    __shared__ LoopLds<PRED> shared;
+    __shared__ LoopLds<PRED, FASTV> shared;
     Lds<PRED> &lds = shared.step;
     init_lds(lds);
     unsigned long long t_begin = s.instrument ? wall_clock64() : 0, busy = 0;
@@ -1916,7 +1993,7 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
                 // frontier kept in LDS between levels
                 if (blockIdx.x == 0) {
                     unsigned long long t0 = s.instrument ? wall_clock64() : 0;
-                    fast_lds_path<PRED, EXACT>(g, s, c, n, loop, vsum, shared.fast, lds_path && !first);
+                    fast_lds_path<PRED, EXACT, FASTV>(g, s, c, n, loop, vsum, shared.fast, lds_path && !first);
                     __syncthreads();
                     init_lds(lds);
                     if (s.instrument) busy += wall_clock64() - t0;
@@ -1975,7 +2052,7 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
         if (blockIdx.x == 0 && threadIdx.x == 0) prepare_step(s.ctl, c, n);
         if (spread) {
             if constexpr (LOOP == kLoopEC || LOOP == kLoopDO) {
-                spread_step<PRED, EXACT>(g, s, c, n, shared.fast, next_bits);
+                spread_step<PRED, EXACT>(g, s, c, n, shared.fast.f, next_bits);
                 __syncthreads();
                 init_lds(lds);
             }
@@ -2172,21 +2249,29 @@ __global__ __launch_bounds__(kWave) void wave_selftest_kernel(const int *in, int
 namespace {
 
 template <bool PRED, bool EXACT>
-const void *persistent_ptr(int loop)
+const void *persistent_ptr(int loop, bool fastv)
 {
     switch (loop) {
-    case kLoopEC: return reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopEC>);
-    case kLoopCE: return reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopCE>);
-    case kLoopTP: return reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopTP>);
-    default: return reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopDO>);
+    case kLoopEC:
+        return fastv ? reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopEC, true>)
+                     : reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopEC, false>);
+    case kLoopCE: return reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopCE, false>);
+    case kLoopTP: return reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopTP, false>);
+    default:
+        return fastv ? reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopDO, true>)
+                     : reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopDO, false>);
     }
 }
 
-const void *persistent_select(int loop, bool pred, bool exact)
+// fastv: the instantiation whose fast path keeps the visited bitmap in LDS
+// (expand-contract and direction-optimizing loops only)
+const void *persistent_select(int loop, bool pred, bool exact, bool fastv)
 {
-    if (pred) return exact ? persistent_ptr<true, true>(loop) : persistent_ptr<true, false>(loop);
-    return exact ? persistent_ptr<false, true>(loop) : persistent_ptr<false, false>(loop);
+    if (pred) return exact ? persistent_ptr<true, true>(loop, fastv) : persistent_ptr<true, false>(loop, fastv);
+    return exact ? persistent_ptr<false, true>(loop, fastv) : persistent_ptr<false, false>(loop, fastv);
 }
+
+bool uses_fast_visited(const SearchArgs &s) { return s.fast_visited && s.nodes <= kFastVisitedWords * 32; }
 
 template <bool PRED, bool EXACT, int PHASE>
 hipError_t launch_phase(const GraphArgs &g, const SearchArgs &s, Cursor c, int grid, hipStream_t st, int *launches)
@@ -2281,15 +2366,22 @@ hipError_t launch_persistent(int loop, bool pred, bool exact, const GraphArgs &g
     SearchArgs sa = s;
     Cursor ca = c;
     void *args[] = {&ga, &sa, &ca};
-    return launch_coresident(persistent_select(loop, pred, exact), dim3(grid), dim3(kBlock), args, st);
+    return launch_coresident(persistent_select(loop, pred, exact, uses_fast_visited(s)), dim3(grid), dim3(kBlock),
+                             args, st);
 }
 
+// The smaller of the two instantiations' occupancies (which one a search
+// launches depends on the graph and the configuration).
 int persistent_max_blocks_per_cu(int loop, bool pred, bool exact)
 {
-    int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, persistent_select(loop, pred, exact), kBlock, 0) !=
-        hipSuccess)
-        return 0;
+    int blocks = 1 << 30;
+    for (int fastv = 0; fastv < 2; fastv++) {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, persistent_select(loop, pred, exact, fastv != 0), kBlock,
+                                                         0) != hipSuccess)
+            return 0;
+        blocks = std::min(blocks, b);
+    }
     return blocks;
 }
 

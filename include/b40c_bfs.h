@@ -128,7 +128,15 @@ typedef struct bfs_config {
                                        2 = a single-pass chained scan over the step's
                                        tiles (decoupled look-back), which keeps the
                                        input order */
+    int32_t fast_visited;           /* visited claims of the single-workgroup fast
+                                       path: 1 = in a copy of the visited bitmap in
+                                       the workgroup's LDS when the graph has at most
+                                       BFS_FAST_VISITED_MAX vertices (else as 2),
+                                       2 = atomics on the bitmap in device memory */
 } bfs_config;
+
+/* Most vertices for which fast_visited = 1 keeps the visited bitmap in LDS. */
+#define BFS_FAST_VISITED_MAX 1103872
 
 typedef struct bfs_stats {
     int64_t total_queued;    /* elements passed through the frontier queues */

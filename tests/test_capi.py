@@ -53,6 +53,21 @@ def test_invalid_arguments_fail_without_device(lib):
     cfg = _native.BfsConfig()
     assert lib.bfs_config_default(ctypes.byref(cfg)) == 0
     assert cfg.wave_gather_threshold > 0 and cfg.cta_gather_threshold >= cfg.wave_gather_threshold
+    assert cfg.fast_visited == 1
+
+
+def test_header_constants_match_binding():
+    import re
+    from back40computing_amd import _native
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "..", "include", "b40c_bfs.h")) as f:
+        text = f.read()
+    m = re.search(r"#define BFS_FAST_VISITED_MAX (\d+)", text)
+    assert m and int(m.group(1)) == _native.FAST_VISITED_MAX
+    # the config struct's fields in declaration order
+    body = text[text.index("typedef struct bfs_config {"):text.index("} bfs_config;")]
+    names = re.findall(r"int32_t (\w+);", body)
+    assert names == [f for f, _ in _native.BfsConfig._fields_]
 
 
 

@@ -290,6 +290,43 @@ def test_spread_levels(gpu, cull, mark_pred):
     p.destroy()
 
 
+@pytest.mark.parametrize("cull", [b40c.CULL_EXACT, b40c.CULL_BEST_EFFORT], ids=["exact", "best_effort"])
+@pytest.mark.parametrize("mark_pred", [False, True], ids=["depth", "pred"])
+def test_fast_path_visited_modes(gpu, cull, mark_pred):
+    # The single-workgroup fast path claims in an LDS copy of the visited
+    # bitmap when the graph fits (fast_visited 1, the default) or with
+    # atomics on the bitmap in device memory (2).  Bitmap lengths that are
+    # not a multiple of four words take the copy's tail loop.
+    graphs = [CsrGraph.grid2d(61), CsrGraph.erdos_renyi(4133, 20000, seed=0xF1),
+              CsrGraph.rmat(14, 8, seed=0x77)]
+    for g in graphs:
+        p = CsrProblem(mark_predecessors=mark_pred).from_host_problem(g)
+        qs = g.edges / g.nodes + 1.0
+        for mode in (1, 2):
+            p.set_config(cull_mode=cull, fast_visited=mode)
+            for src in g.sample_sources(2, seed=9):
+                assert_parity(g, p, b40c.EXPAND_CONTRACT, True, src, queue_sizing=qs)
+                assert_parity(g, p, b40c.DIRECTION_OPTIMIZING, True, src, queue_sizing=qs)
+        p.destroy()
+
+
+@pytest.mark.slow
+def test_fast_path_visited_size_limit(gpu):
+    # the largest lattice whose visited bitmap fits the fast path's LDS copy
+    # and the first that does not; labels by the closed form d = i + j
+    assert 1050 * 1050 <= b40c._native.FAST_VISITED_MAX < 1051 * 1051
+    for side in (1050, 1051):
+        g = CsrGraph.grid2d(side)
+        j, k = np.divmod(np.arange(g.nodes), side)
+        expect = (j + k).astype(np.int32)
+        p = CsrProblem().from_host_problem(g)
+        for strategy in (b40c.EXPAND_CONTRACT, b40c.DIRECTION_OPTIMIZING):
+            got, st = run(p, strategy, True, 0)
+            assert oracle.compare(got, expect) == -1, (side, strategy)
+            assert st.search_depth == 2 * side - 1
+        p.destroy()
+
+
 def test_invalid_source_and_frontier_type(gpu):
     g = CsrGraph.grid2d(4)
     p = CsrProblem().from_host_problem(g)
