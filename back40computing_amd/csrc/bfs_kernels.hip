@@ -621,7 +621,7 @@ __device__ __forceinline__ void queue_from_bitmap(const GraphArgs &g, const Sear
 }
 
 template <bool PRED, bool EXACT>
-__device__ void body_contract_expand(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds,
+__device__ __forceinline__ void body_contract_expand(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds,
                                      int sub, bool hubs)
 {
     WaveQueue<PRED> wq = make_queue<PRED>(s, c.step, lds);
@@ -649,7 +649,7 @@ __device__ void body_contract_expand(const GraphArgs &g, const SearchArgs &s, Cu
 }
 
 template <bool PRED, bool EXACT>
-__device__ void body_expand(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds, int sub,
+__device__ __forceinline__ void body_expand(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds, int sub,
                             bool hubs)
 {
     WaveQueue<PRED> wq = make_queue<PRED>(s, c.step, lds);
@@ -872,7 +872,9 @@ __device__ __forceinline__ void loop_body(int phase, const GraphArgs &g, const S
     if constexpr (LOOP == kLoopEC) {
         body_expand_contract<PRED, EXACT>(g, s, c, n, lds, sub, hubs);
     } else if constexpr (LOOP == kLoopCE) {
-        body_contract_expand<PRED, EXACT>(g, s, c, n, lds, sub, hubs);
+        // (expand: the step that ends a contract-expand fast path)
+        if (phase == kPhaseExpand) body_expand<PRED, EXACT>(g, s, c, n, lds, sub, hubs);
+        else body_contract_expand<PRED, EXACT>(g, s, c, n, lds, sub, hubs);
     } else if constexpr (LOOP == kLoopDO) {
         if (phase == kPhaseBottomUp) body_bottom_up<PRED, EXACT>(g, s, c, lds);
         else body_expand_contract<PRED, EXACT>(g, s, c, n, lds, sub, hubs, next_bits, from_bitmap);
@@ -1204,7 +1206,11 @@ __device__ __forceinline__ Cursor advance(int loop, Cursor c)
 {
     c.step++;
     if (loop != kLoopTP) {
+        // (a contract-expand loop returns to contract-expand after the
+        // expand step that ends its fast path: the expanded members were
+        // labelled with c.iteration, their neighbours get c.iteration + 1)
         c.iteration++;
+        if (loop == kLoopCE) c.phase = kPhaseContractExpand;
     } else {
         if (c.phase == kPhaseContract) c.phase = kPhaseExpand;
         else if (c.phase == kPhaseExpand) c.phase = kPhaseFilter;
@@ -1284,9 +1290,9 @@ struct FastFront {
 // FASTV: the fast path's copy of the visited bitmap takes the rest of the
 // workgroup's LDS (gfx950: 160 KiB) once the frontier buffers and 1 KiB for
 // the level loop's other variables are placed: graphs of up to
-// kFastVisitedWords * 32 vertices.  Only the expand-contract and
-// direction-optimizing loops have an instantiation with it; the others (and
-// larger graphs) keep the smaller LDS footprint.
+// kFastVisitedWords * 32 vertices.  Only the expand-contract,
+// contract-expand and direction-optimizing loops have an instantiation with
+// it; two-phase (and larger graphs) keep the smaller LDS footprint.
 constexpr int kLdsBytes = 163840;
 constexpr int kFastVisitedWords = ((kLdsBytes - 1024 - (int) sizeof(FastFront)) / 4) & ~63;
 static_assert(kFastVisitedWords * 32 == BFS_FAST_VISITED_MAX, "the header states the fast path's LDS bitmap size");
@@ -1345,15 +1351,12 @@ __device__ int fast_scan(FastFront &F, int b, unsigned n)
 // only) the claims are LDS atomics on the path's copy of the visited
 // bitmap: no pre-read and no memory trip between a column entry and the
 // claimed vertex's row bounds.
+// fast_claim_vertices: the same for vertices u[k] (-1: none) already at hand.
 template <bool PRED, bool EXACT, bool LOCAL, int S>
-__device__ __forceinline__ unsigned fast_claim_slots(const GraphArgs &g, const SearchArgs &s, int next_depth,
-                                                     const int (&at)[S], const int (&pv)[S], int (&u)[S],
-                                                     int (&ub)[S], int (&ud)[S], bool ldsv = false,
-                                                     unsigned *vbits = nullptr)
+__device__ __forceinline__ unsigned fast_claim_vertices(const GraphArgs &g, const SearchArgs &s, int next_depth,
+                                                        const int (&pv)[S], const int (&u)[S], int (&ub)[S],
+                                                        int (&ud)[S], bool ldsv, unsigned *vbits)
 {
-#pragma unroll
-    for (int k = 0; k < S; k++)
-        u[k] = at[k] < 0 ? -1 : LOCAL ? g.col[at[k]] : __builtin_nontemporal_load(g.col + at[k]);
     unsigned old[S];
     if (LOCAL && ldsv) {
 #pragma unroll
@@ -1422,6 +1425,18 @@ __device__ __forceinline__ unsigned fast_claim_slots(const GraphArgs &g, const S
         }
     }
     return fresh;
+}
+
+template <bool PRED, bool EXACT, bool LOCAL, int S>
+__device__ __forceinline__ unsigned fast_claim_slots(const GraphArgs &g, const SearchArgs &s, int next_depth,
+                                                     const int (&at)[S], const int (&pv)[S], int (&u)[S],
+                                                     int (&ub)[S], int (&ud)[S], bool ldsv = false,
+                                                     unsigned *vbits = nullptr)
+{
+#pragma unroll
+    for (int k = 0; k < S; k++)
+        u[k] = at[k] < 0 ? -1 : LOCAL ? g.col[at[k]] : __builtin_nontemporal_load(g.col + at[k]);
+    return fast_claim_vertices<PRED, EXACT, LOCAL, S>(g, s, next_depth, pv, u, ub, ud, ldsv, vbits);
 }
 
 // kFastK consecutive edges from e0 (one thread's share of a batch) of an
@@ -1497,6 +1512,37 @@ __device__ __forceinline__ void fast_append(const SearchArgs &s, FastFront &F, i
     }
 }
 
+// Folds the threads' edge sums, label flags and largest degrees of the
+// members just appended to buffer dst into F.fe, F.labeled, F.maxdeg[dst].
+// Every thread calls it.
+__device__ __forceinline__ void fast_fold(FastFront &F, int dst, unsigned long long fe, bool labeled, int dmax)
+{
+    const int lane = lane_id(), wave = threadIdx.x / kWave;
+    const int lo = wave_inclusive_sum((int) (fe & 0xFFFFu));
+    const int hi = wave_inclusive_sum((int) (fe >> 16));
+    const int wmax = wave_inclusive_max_nonneg(dmax);
+    const bool any = ballot(labeled) != 0;
+    if (lane == kWave - 1) {
+        F.wfe[wave] = ((unsigned long long) (unsigned) hi << 16) + (unsigned long long) (unsigned) lo;
+        F.wlab[wave] = any ? 1 : 0;
+        F.wmax[wave] = wmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long f = 0;
+        int l = 0, dm = 0;
+        for (int w = 0; w < kWaves; w++) {
+            f += F.wfe[w];
+            l |= F.wlab[w];
+            dm = max(dm, F.wmax[w]);
+        }
+        F.fe = f;
+        F.labeled = l;
+        F.maxdeg[dst] = dm;
+    }
+    __syncthreads();
+}
+
 // One level from LDS buffer src (n members, 0 < n <= kFastCap) into buffer
 // src ^ 1; members past kFastCap go to the global queue of step c + 1 at the
 // same positions.  When the members' rows fit kFastDirect slots per thread
@@ -1509,7 +1555,6 @@ __device__ __forceinline__ void fast_lds_level(const GraphArgs &g, const SearchA
                                                FastFront &F, bool ldsv, unsigned *vbits)
 {
     const int dst = src ^ 1;
-    const int lane = lane_id(), wave = threadIdx.x / kWave;
     const int next_depth = c.iteration + 1;
     int *spill = s.q[(c.step + 1) & 1];
     const int maxd = F.maxdeg[src];
@@ -1552,30 +1597,7 @@ __device__ __forceinline__ void fast_lds_level(const GraphArgs &g, const SearchA
             fast_append<kFastK>(s, F, dst, spill, fresh, u, ub, ud, fe, labeled, dmax);
         }
     }
-    // fold the threads' edge sums, label flags and largest degrees
-    const int lo = wave_inclusive_sum((int) (fe & 0xFFFFu));
-    const int hi = wave_inclusive_sum((int) (fe >> 16));
-    const int wmax = wave_inclusive_max_nonneg(dmax);
-    const bool any = ballot(labeled) != 0;
-    if (lane == kWave - 1) {
-        F.wfe[wave] = ((unsigned long long) (unsigned) hi << 16) + (unsigned long long) (unsigned) lo;
-        F.wlab[wave] = any ? 1 : 0;
-        F.wmax[wave] = wmax;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long f = 0;
-        int l = 0, dm = 0;
-        for (int w = 0; w < kWaves; w++) {
-            f += F.wfe[w];
-            l |= F.wlab[w];
-            dm = max(dm, F.wmax[w]);
-        }
-        F.fe = f;
-        F.labeled = l;
-        F.maxdeg[dst] = dm;
-    }
-    __syncthreads();
+    fast_fold(F, dst, fe, labeled, dmax);
 }
 
 // Spread step: a top-down level of few members (n <= kFastCap) with more
@@ -1710,36 +1732,78 @@ __device__ __forceinline__ void fast_lds_path(const GraphArgs &g, const SearchAr
         }
         for (int i = 4 * chunks + (int) threadIdx.x; i < vwords; i += kBlock) L.vbits[i] = s.visited[i];
     }
-    int dmax = 0;
-    for (unsigned i = threadIdx.x; i < n; i += kBlock) {
-        const int v = ld_queue(in + i);
-        const int b = g.row[v];
-        const int d = g.row[v + 1] - b;
-        F.v[0][i] = v;
-        F.beg[0][i] = b;
-        F.deg[0][i] = d;
-        dmax = max(dmax, d);
-    }
-    dmax = wave_inclusive_max_nonneg(dmax);
-    if (lane_id() == kWave - 1) F.wmax[threadIdx.x / kWave] = dmax;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int dm = 0;
-        for (int w = 0; w < kWaves; w++) dm = max(dm, F.wmax[w]);
-        F.maxdeg[0] = dm;
-    }
-    __syncthreads();
     int src = 0;
     unsigned long long ve_add = 0, queued = 0;
     int maxd = -1;
     unsigned n2 = 0;
     unsigned long long fe2 = 0;
-    while (true) {
-        if (threadIdx.x == 0 && c.step < kLevelLog) {
-            s.ctl->level_len[c.step] = n;
-            s.ctl->level_clock[c.step] = wall_clock64();
+    const bool ce = loop == kLoopCE;
+    if (ce) {
+        // Contract-expand: step c's input is an edge frontier (the previous
+        // level's neighbours, with their parents).  Its claims, labelled
+        // with step c's iteration as the contraction of step c labels them,
+        // become the LDS members; each level below then does the next
+        // step's contraction, and a step's input length (the frontier edges
+        // of the members before it) is what total_queued counts.
+        if (threadIdx.x == 0) {
+            F.count = 0;
+            if (c.step < kLevelLog) {
+                s.ctl->level_len[c.step] = n;
+                s.ctl->level_clock[c.step] = wall_clock64();
+            }
         }
+        __syncthreads();
         queued += n;
+        const int *pin = (c.step & 1) ? s.pq[1] : s.pq[0];
+        static_assert(kFastCap <= 2 * kBlock, "two input items per thread");
+        int u[2], pv[2], ub[2], ud[2];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const unsigned i = threadIdx.x + (unsigned) k * kBlock;
+            u[k] = i < n ? ld_queue(in + i) : -1;
+            pv[k] = PRED && i < n ? ld_queue(pin + i) : -1;
+        }
+        const unsigned fresh =
+            fast_claim_vertices<PRED, EXACT, true, 2>(g, s, c.iteration, pv, u, ub, ud, ldsv, L.vbits);
+        unsigned long long fe = 0;
+        bool labeled = false;
+        int dmax = 0;
+        fast_append<2>(s, F, 0, (c.step & 1) ? s.q[0] : s.q[1], fresh, u, ub, ud, fe, labeled, dmax);
+        fast_fold(F, 0, fe, labeled, dmax);
+        n = F.count;  // (at most the input's n <= kFastCap: nothing spilled)
+        fe2 = F.fe;
+        if (F.labeled) maxd = c.iteration;
+        if (n == 0) c = advance(loop, c);  // the next step's input is empty
+    } else {
+        int dmax = 0;
+        for (unsigned i = threadIdx.x; i < n; i += kBlock) {
+            const int v = ld_queue(in + i);
+            const int b = g.row[v];
+            const int d = g.row[v + 1] - b;
+            F.v[0][i] = v;
+            F.beg[0][i] = b;
+            F.deg[0][i] = d;
+            dmax = max(dmax, d);
+        }
+        dmax = wave_inclusive_max_nonneg(dmax);
+        if (lane_id() == kWave - 1) F.wmax[threadIdx.x / kWave] = dmax;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int dm = 0;
+            for (int w = 0; w < kWaves; w++) dm = max(dm, F.wmax[w]);
+            F.maxdeg[0] = dm;
+        }
+        __syncthreads();
+    }
+    while (n > 0) {
+        // (a contract-expand level is the contraction of step c + 1)
+        const int logged = c.step + (ce ? 1 : 0);
+        const unsigned long long step_input = ce ? fe2 : n;
+        if (threadIdx.x == 0 && logged < kLevelLog) {
+            s.ctl->level_len[logged] = (unsigned) step_input;
+            s.ctl->level_clock[logged] = wall_clock64();
+        }
+        queued += step_input;
         fast_lds_level<PRED, EXACT>(g, s, c, n, src, F, ldsv, L.vbits);
         n2 = F.count;
         fe2 = F.fe;
@@ -1753,9 +1817,20 @@ __device__ __forceinline__ void fast_lds_path(const GraphArgs &g, const SearchAr
             if (direction_rule(s, c.phase, fe2, n2, fv2) != c.phase) go = false;
             else fv = fv2;
         }
+        // a hybrid search leaves contract-expand when a step's input
+        // exceeds the saturation threshold: the grid takes that step
+        if (go && ce && s.sat_t >= 0 && fe2 > (unsigned long long) s.sat_t) go = false;
         n = n2;
         if (!go) break;
         __syncthreads();  // every thread has read F's results before the next level resets them
+    }
+    // Contract-expand hands its members to the grid as the input of an
+    // expand step (cursor c, phase expand: their neighbours become step
+    // c + 1's edge frontier), whose input the loop top adds to total_queued
+    // although it is not a contract-expand step's.
+    if (ce && n2 > 0) {
+        c.phase = kPhaseExpand;
+        queued -= min(n2, s.cap);
     }
     if (ldsv) {
         // (the last level's claims precede its closing workgroup barrier)
@@ -1924,7 +1999,8 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
             if (n == 0) { reason = kExitDone; break; }
             if (s.track_edges) fe = ld64_agent(&s.ctl->fedges[c.step % 3]);
         }
-        if (loop == kLoopCE && s.sat_t >= 0 && n > (unsigned) s.sat_t) {
+        // (an expand step that ends a contract-expand fast path runs first)
+        if (loop == kLoopCE && c.phase != kPhaseExpand && s.sat_t >= 0 && n > (unsigned) s.sat_t) {
             reason = kExitSaturated;
             break;
         }
@@ -1951,7 +2027,11 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
             c.phase = dir;
         }
 
-        const bool fast = fast_level(s, c, n, fe) && gridDim.x > 1 &&
+        // Contract-expand takes the LDS path from a contract-expand step of
+        // at most kFastCap items (never from the expand step it hands over
+        // to the grid with).
+        const bool ce_lds = LOOP == kLoopCE && c.phase == kPhaseContractExpand && n <= (unsigned) kFastCap;
+        const bool fast = fast_level(s, c, n, fe) && gridDim.x > 1 && !(LOOP == kLoopCE && c.phase == kPhaseExpand) &&
                           ((LOOP != kLoopEC && LOOP != kLoopDO) || (n <= (unsigned) kFastCap && fe <= kFastEdges));
         // large top-down levels mark instead of claiming (grid-uniform: fe
         // is this step's frontier-edge count); few-member ones spread their
@@ -1987,9 +2067,9 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
             // waits for the others' arrival at the closing barrier -- except
             // after a folded search start, whose label rewrite the entry
             // barrier orders before the first claim.
-            const bool lds_path = LOOP == kLoopEC || LOOP == kLoopDO;
+            const bool lds_path = LOOP == kLoopEC || LOOP == kLoopDO || ce_lds;
             if (!lds_path || first) grid_barrier(s.ctl);
-            if constexpr (LOOP == kLoopEC || LOOP == kLoopDO) {
+            if (lds_path) {
                 // frontier kept in LDS between levels
                 if (blockIdx.x == 0) {
                     unsigned long long t0 = s.instrument ? wall_clock64() : 0;
@@ -2255,7 +2335,9 @@ const void *persistent_ptr(int loop, bool fastv)
     case kLoopEC:
         return fastv ? reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopEC, true>)
                      : reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopEC, false>);
-    case kLoopCE: return reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopCE, false>);
+    case kLoopCE:
+        return fastv ? reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopCE, true>)
+                     : reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopCE, false>);
     case kLoopTP: return reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopTP, false>);
     default:
         return fastv ? reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopDO, true>)
@@ -2264,7 +2346,7 @@ const void *persistent_ptr(int loop, bool fastv)
 }
 
 // fastv: the instantiation whose fast path keeps the visited bitmap in LDS
-// (expand-contract and direction-optimizing loops only)
+// (expand-contract, contract-expand and direction-optimizing loops)
 const void *persistent_select(int loop, bool pred, bool exact, bool fastv)
 {
     if (pred) return exact ? persistent_ptr<true, true>(loop, fastv) : persistent_ptr<true, false>(loop, fastv);
