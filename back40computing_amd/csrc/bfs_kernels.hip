@@ -1272,6 +1272,20 @@ static_assert(kFastCap <= 4 * kBlock, "the degree scan takes four members per th
 // whole rows (a thread up to kFastDirect edges) with no degree scan and no
 // search for each edge's member.
 constexpr int kFastDirect = 8;
+static_assert(kFastDirect == 8, "fast_direct_lg spans one to eight lanes per member");
+
+// log2 of the lanes per member of a direct level whose members' largest
+// degree is maxd (30: too large for one)
+__device__ __forceinline__ int fast_direct_lg(int maxd)
+{
+    return maxd <= 1 ? 0 : maxd <= 2 ? 1 : maxd <= 4 ? 2 : maxd <= kFastDirect ? 3 : 30;
+}
+
+__device__ __forceinline__ bool fast_direct(int maxd, unsigned n)
+{
+    const int lg = fast_direct_lg(maxd);
+    return lg < 30 && ((int) n << lg) <= kBlock * kFastDirect;
+}
 
 struct FastFront {
     int v[2][kFastCap];
@@ -1279,6 +1293,7 @@ struct FastFront {
     int deg[2][kFastCap];        // degree on append; the level's exclusive scan in place
     int maxdeg[2];               // largest degree of each buffer's members
     unsigned count;              // next-frontier appends
+    unsigned dups;               // two-phase: further edges to vertices claimed in the level
     unsigned long long fe;       // next frontier's edges
     int labeled;
     int scan[kWaves];            // per-wave sums of the degree scan
@@ -1290,9 +1305,8 @@ struct FastFront {
 // FASTV: the fast path's copy of the visited bitmap takes the rest of the
 // workgroup's LDS (gfx950: 160 KiB) once the frontier buffers and 1 KiB for
 // the level loop's other variables are placed: graphs of up to
-// kFastVisitedWords * 32 vertices.  Only the expand-contract,
-// contract-expand and direction-optimizing loops have an instantiation with
-// it; two-phase (and larger graphs) keep the smaller LDS footprint.
+// kFastVisitedWords * 32 vertices.  Larger graphs launch the instantiation
+// without it, which keeps the smaller LDS footprint.
 constexpr int kLdsBytes = 163840;
 constexpr int kFastVisitedWords = ((kLdsBytes - 1024 - (int) sizeof(FastFront)) / 4) & ~63;
 static_assert(kFastVisitedWords * 32 == BFS_FAST_VISITED_MAX, "the header states the fast path's LDS bitmap size");
@@ -1544,21 +1558,26 @@ __device__ __forceinline__ void fast_fold(FastFront &F, int dst, unsigned long l
 }
 
 // One level from LDS buffer src (n members, 0 < n <= kFastCap) into buffer
-// src ^ 1; members past kFastCap go to the global queue of step c + 1 at the
-// same positions.  When the members' rows fit kFastDirect slots per thread
+// src ^ 1; members past kFastCap go to the global queue `spill` (the one the
+// step after the path reads) at the same positions.  When the members' rows fit kFastDirect slots per thread
 // (every lattice level), each member's row goes to a group of consecutive
 // lanes, one edge per lane; otherwise the level's edges are spread evenly
 // over the threads through the degree scan, kFastK per thread per batch.
 // Every thread calls it.
+// count_dups (direct levels only): also counts in F.dups the edges, other
+// than the claiming ones, whose targets the level claimed.
 template <bool PRED, bool EXACT>
 __device__ __forceinline__ void fast_lds_level(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, int src,
-                                               FastFront &F, bool ldsv, unsigned *vbits)
+                                               FastFront &F, bool ldsv, unsigned *vbits, int *spill,
+                                               bool count_dups = false)
 {
     const int dst = src ^ 1;
     const int next_depth = c.iteration + 1;
-    int *spill = s.q[(c.step + 1) & 1];
     const int maxd = F.maxdeg[src];
-    if (threadIdx.x == 0) F.count = 0;
+    if (threadIdx.x == 0) {
+        F.count = 0;
+        F.dups = 0;
+    }
     unsigned long long fe = 0;
     bool labeled = false;
     int dmax = 0;
@@ -1566,9 +1585,8 @@ __device__ __forceinline__ void fast_lds_level(const GraphArgs &g, const SearchA
     // degree), lane k of a group taking edge k, so that the column entries
     // one load instruction reads are whole rows side by side; kFastDirect
     // such passes cover up to kBlock * kFastDirect / 2^lg members.
-    const int lg = maxd <= 1 ? 0 : maxd <= 2 ? 1 : maxd <= 4 ? 2 : maxd <= kFastDirect ? 3 : 30;
-    static_assert(kFastDirect == 8, "lg spans one to eight lanes per member");
-    if (lg < 30 && ((int) n << lg) <= kBlock * kFastDirect) {
+    const int lg = fast_direct_lg(maxd);
+    if (fast_direct(maxd, n)) {
         __syncthreads();  // the resets above precede every append
         int at[kFastDirect], pv[kFastDirect];
         const int k = (int) threadIdx.x & ((1 << lg) - 1);
@@ -1587,6 +1605,22 @@ __device__ __forceinline__ void fast_lds_level(const GraphArgs &g, const SearchA
         const unsigned fresh =
             fast_claim_slots<PRED, EXACT, true, kFastDirect>(g, s, next_depth, at, pv, u, ub, ud, ldsv, vbits);
         fast_append<kFastDirect>(s, F, dst, spill, fresh, u, ub, ud, fe, labeled, dmax);
+        if (count_dups) {
+            fast_fold(F, dst, fe, labeled, dmax);
+            // (after the fold's barriers every label of the level is
+            // visible to the workgroup)
+            int dep[kFastDirect];
+#pragma unroll
+            for (int j = 0; j < kFastDirect; j++)
+                dep[j] = u[j] >= 0 && !((fresh >> j) & 1) ? s.depth[u[j]] : -1;
+            int d = 0;
+#pragma unroll
+            for (int j = 0; j < kFastDirect; j++) d += dep[j] == next_depth ? 1 : 0;
+            const int wsum = wave_inclusive_sum(d);
+            if (lane_id() == kWave - 1 && wsum) atomicAdd(&F.dups, (unsigned) wsum);
+            __syncthreads();
+            return;
+        }
     } else {
         const int total = fast_scan(F, src, n);  // (its barriers order the resets before the appends)
         for (int b0 = 0; b0 < total; b0 += kBlock * kFastK) {
@@ -1737,14 +1771,17 @@ __device__ __forceinline__ void fast_lds_path(const GraphArgs &g, const SearchAr
     int maxd = -1;
     unsigned n2 = 0;
     unsigned long long fe2 = 0;
-    const bool ce = loop == kLoopCE;
-    if (ce) {
-        // Contract-expand: step c's input is an edge frontier (the previous
-        // level's neighbours, with their parents).  Its claims, labelled
-        // with step c's iteration as the contraction of step c labels them,
-        // become the LDS members; each level below then does the next
-        // step's contraction, and a step's input length (the frontier edges
-        // of the members before it) is what total_queued counts.
+    const bool ce = loop == kLoopCE, tp = loop == kLoopTP;
+    if (ce || tp) {
+        // Contract-expand and two-phase enter at a step that contracts a
+        // list of candidates (the previous level's neighbours or their
+        // filtered remainder, with their parents).  Its claims, labelled
+        // with the step's iteration as the step labels them, become the LDS
+        // members; each level below then does the next contraction.
+        // Contract-expand counts a step's input (the frontier edges of the
+        // members before it) in total_queued; two-phase counts an expand
+        // (the members), a filter (their edges) and a contraction (the
+        // edges to vertices the level claims).
         if (threadIdx.x == 0) {
             F.count = 0;
             if (c.step < kLevelLog) {
@@ -1796,21 +1833,37 @@ __device__ __forceinline__ void fast_lds_path(const GraphArgs &g, const SearchAr
         __syncthreads();
     }
     while (n > 0) {
+        // Two-phase counts a contraction's input -- every edge the filter
+        // passes, i.e. every edge to a vertex the level claims -- which
+        // direct levels count; the grid takes the others.
+        if (tp && !fast_direct(F.maxdeg[src], n)) {
+            n2 = n;
+            break;
+        }
         // (a contract-expand level is the contraction of step c + 1)
         const int logged = c.step + (ce ? 1 : 0);
-        const unsigned long long step_input = ce ? fe2 : n;
+        const unsigned long long step_input = ce ? fe2 : tp ? n + fe2 : n;
         if (threadIdx.x == 0 && logged < kLevelLog) {
             s.ctl->level_len[logged] = (unsigned) step_input;
             s.ctl->level_clock[logged] = wall_clock64();
         }
         queued += step_input;
-        fast_lds_level<PRED, EXACT>(g, s, c, n, src, F, ldsv, L.vbits);
+        // the queue the step after the path reads if this level ends it
+        // (members past kFastCap go there): that of step c + 1, or of the
+        // expand step four steps on from a two-phase contraction
+        int *spill = tp ? s.q[c.step & 1] : s.q[(c.step + 1) & 1];
+        fast_lds_level<PRED, EXACT>(g, s, c, n, src, F, ldsv, L.vbits, spill, tp);
         n2 = F.count;
         fe2 = F.fe;
         if (F.labeled) maxd = c.iteration + 1;
         ve_add += fe2;
         src ^= 1;
         c = advance(loop, c);
+        if (tp) {
+            // expand and filter, then the contraction of the filter's output
+            queued += (unsigned long long) n2 + F.dups;
+            c = advance(loop, advance(loop, c));
+        }
         bool go = n2 > 0 && n2 <= (unsigned) kFastCap && n2 <= (unsigned) s.fast_t && fe2 <= kFastEdges;
         if (go && loop == kLoopDO) {
             const unsigned long long fv2 = fv + fe2;
@@ -1832,6 +1885,9 @@ __device__ __forceinline__ void fast_lds_path(const GraphArgs &g, const SearchAr
         c.phase = kPhaseExpand;
         queued -= min(n2, s.cap);
     }
+    // Two-phase hands its members to the expand step after their
+    // contraction, which counts them as its input.
+    if (tp && n2 > 0) c = advance(loop, c);
     if (ldsv) {
         // (the last level's claims precede its closing workgroup barrier)
         const u32x4 *from = reinterpret_cast<const u32x4 *>(L.vbits);
@@ -2027,10 +2083,12 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
             c.phase = dir;
         }
 
-        // Contract-expand takes the LDS path from a contract-expand step of
-        // at most kFastCap items (never from the expand step it hands over
-        // to the grid with).
-        const bool ce_lds = LOOP == kLoopCE && c.phase == kPhaseContractExpand && n <= (unsigned) kFastCap;
+        // Contract-expand and two-phase take the LDS path from a step that
+        // contracts at most kFastCap items (never from the expand step the
+        // path hands over to the grid with).
+        const bool contract_lds = ((LOOP == kLoopCE && c.phase == kPhaseContractExpand) ||
+                                   (LOOP == kLoopTP && c.phase == kPhaseContract)) &&
+                                  n <= (unsigned) kFastCap;
         const bool fast = fast_level(s, c, n, fe) && gridDim.x > 1 && !(LOOP == kLoopCE && c.phase == kPhaseExpand) &&
                           ((LOOP != kLoopEC && LOOP != kLoopDO) || (n <= (unsigned) kFastCap && fe <= kFastEdges));
         // large top-down levels mark instead of claiming (grid-uniform: fe
@@ -2067,7 +2125,7 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
             // waits for the others' arrival at the closing barrier -- except
             // after a folded search start, whose label rewrite the entry
             // barrier orders before the first claim.
-            const bool lds_path = LOOP == kLoopEC || LOOP == kLoopDO || ce_lds;
+            const bool lds_path = LOOP == kLoopEC || LOOP == kLoopDO || contract_lds;
             if (!lds_path || first) grid_barrier(s.ctl);
             if (lds_path) {
                 // frontier kept in LDS between levels
@@ -2095,6 +2153,8 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
                     n = lds.nb;
                     if (n == 0 || !fast_level(s, c, n) || ld_agent(&s.ctl->ovf[c.step % 3])) break;
                     if (loop == kLoopCE && s.sat_t >= 0 && n > (unsigned) s.sat_t) break;
+                    // (a small two-phase contraction is the LDS path's entry)
+                    if (loop == kLoopTP && c.phase == kPhaseContract && n <= (unsigned) kFastCap) break;
                     if (loop == kLoopDO) {
                         unsigned long long fv2 = fv + ld64_agent(&s.ctl->fedges[c.step % 3]);
                         if (choose_direction(s, c, n, fv2) != c.phase) break;
@@ -2338,7 +2398,9 @@ const void *persistent_ptr(int loop, bool fastv)
     case kLoopCE:
         return fastv ? reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopCE, true>)
                      : reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopCE, false>);
-    case kLoopTP: return reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopTP, false>);
+    case kLoopTP:
+        return fastv ? reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopTP, true>)
+                     : reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopTP, false>);
     default:
         return fastv ? reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopDO, true>)
                      : reinterpret_cast<const void *>(&persistent_kernel<PRED, EXACT, kLoopDO, false>);
@@ -2346,7 +2408,6 @@ const void *persistent_ptr(int loop, bool fastv)
 }
 
 // fastv: the instantiation whose fast path keeps the visited bitmap in LDS
-// (expand-contract, contract-expand and direction-optimizing loops)
 const void *persistent_select(int loop, bool pred, bool exact, bool fastv)
 {
     if (pred) return exact ? persistent_ptr<true, true>(loop, fastv) : persistent_ptr<true, false>(loop, fastv);

@@ -297,16 +297,23 @@ def test_fast_path_visited_modes(gpu, cull, mark_pred):
     # bitmap when the graph fits (fast_visited 1, the default) or with
     # atomics on the bitmap in device memory (2).  Bitmap lengths that are
     # not a multiple of four words take the copy's tail loop.
+    # Contract-expand, two-phase and the hybrid enter the path at a
+    # contraction and hand it back through an expand step; R-MAT's growing
+    # frontier makes them leave it mid-search.
     graphs = [CsrGraph.grid2d(61), CsrGraph.erdos_renyi(4133, 20000, seed=0xF1),
               CsrGraph.rmat(14, 8, seed=0x77)]
     for g in graphs:
         p = CsrProblem(mark_predecessors=mark_pred).from_host_problem(g)
         qs = g.edges / g.nodes + 1.0
+        # (best-effort grid steps let duplicates through: see test_rmat_scale14)
+        qs_edges = 2.0 if cull == b40c.CULL_EXACT else 64.0
         for mode in (1, 2):
             p.set_config(cull_mode=cull, fast_visited=mode)
             for src in g.sample_sources(2, seed=9):
                 assert_parity(g, p, b40c.EXPAND_CONTRACT, True, src, queue_sizing=qs)
                 assert_parity(g, p, b40c.DIRECTION_OPTIMIZING, True, src, queue_sizing=qs)
+                for strategy in (b40c.CONTRACT_EXPAND, b40c.TWO_PHASE, b40c.HYBRID):
+                    assert_parity(g, p, strategy, True, src, queue_sizing=qs_edges)
         p.destroy()
 
 
@@ -320,7 +327,8 @@ def test_fast_path_visited_size_limit(gpu):
         j, k = np.divmod(np.arange(g.nodes), side)
         expect = (j + k).astype(np.int32)
         p = CsrProblem().from_host_problem(g)
-        for strategy in (b40c.EXPAND_CONTRACT, b40c.DIRECTION_OPTIMIZING):
+        for strategy in (b40c.EXPAND_CONTRACT, b40c.DIRECTION_OPTIMIZING, b40c.CONTRACT_EXPAND,
+                         b40c.TWO_PHASE, b40c.HYBRID):
             got, st = run(p, strategy, True, 0)
             assert oracle.compare(got, expect) == -1, (side, strategy)
             assert st.search_depth == 2 * side - 1
