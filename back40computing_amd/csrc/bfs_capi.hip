@@ -262,6 +262,7 @@ bfs_status enact_direction_iterative(bfs_problem *p, const GraphArgs &g, const S
 {
     const bool exact = p->cfg.cull_mode == BFS_CULL_EXACT;
     Cursor c{0, 0, kPhaseExpandContract};
+    unsigned long long prev_mf = 0;  // the previous frontier's edges
     while (true) {
         bfs_status st = read_control(p);
         if (st != BFS_OK) return st;
@@ -270,12 +271,14 @@ bfs_status enact_direction_iterative(bfs_problem *p, const GraphArgs &g, const S
         if (h.overflow || n == 0) return BFS_OK;
         unsigned long long mf = h.fedges[c.step % 3];
         unsigned long long mu = s.total_edges > h.visited_edges ? s.total_edges - h.visited_edges : 0ull;
+        // the persistent loop's rule (bottom-up only while the frontier grows)
         int dir = c.phase;
         if (c.phase == kPhaseExpandContract) {
-            if (mf * (unsigned long long) s.do_alpha > mu) dir = kPhaseBottomUp;
+            if (mf > prev_mf && mf * (unsigned long long) s.do_alpha > mu) dir = kPhaseBottomUp;
         } else if ((unsigned long long) n * (unsigned long long) s.do_beta < (unsigned long long) s.nodes) {
             dir = kPhaseExpandContract;
         }
+        prev_mf = mf;
         if (dir == kPhaseBottomUp && c.phase != kPhaseBottomUp)
             HIP_TRY(launch_build_frontier_bitmap(s, c, grid, p->stream, launches), "build frontier bitmap");
         else if (dir != kPhaseBottomUp && c.phase == kPhaseBottomUp)

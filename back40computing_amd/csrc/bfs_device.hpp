@@ -53,7 +53,8 @@ struct alignas(256) Control {
     unsigned long long visited_edges; // sum of degrees of all vertices labelled so far
     unsigned long long fast_vsum;     // fast-path hand-off of the per-workgroup visited-edge sum
     unsigned ovf[4];                  // per-step overflow of the output queue, by step % 3
-    unsigned long long pad2[9];
+    unsigned long long fast_prev_fe;  // fast-path hand-off: frontier edges of the level before the next step's
+    unsigned long long pad2[8];
     unsigned bar_root;        // own 128-byte line
     unsigned pad3[31];
     unsigned bar_gen;         // completed barriers (own 128-byte line)

@@ -1170,23 +1170,23 @@ __device__ __forceinline__ void mark_compact(const GraphArgs &g, const SearchArg
 // could choose the other direction -- and then a different sequence of grid
 // barriers than the rest of the grid.
 // Direction rule: mf = the frontier's edges, n = its vertices, ve = edges
-// visited so far.
+// visited so far, prev_mf = the previous frontier's edges.  A top-down
+// search turns bottom-up only while its frontier grows (Beamer et al.'s
+// growing phase): near the end of a high-diameter search the unexplored
+// edges fall below alpha times a shrinking frontier's, and a bottom-up
+// level there would turn straight back.
 __device__ __forceinline__ int direction_rule(const SearchArgs &s, int phase, unsigned long long mf, unsigned n,
-                                              unsigned long long ve)
+                                              unsigned long long ve, unsigned long long prev_mf)
 {
     unsigned long long mu = s.total_edges > ve ? s.total_edges - ve : 0ull;
     if (phase == kPhaseExpandContract) {
-        if (mf * (unsigned long long) s.do_alpha > mu) return kPhaseBottomUp;
+        if (mf > prev_mf && mf * (unsigned long long) s.do_alpha > mu) return kPhaseBottomUp;
     } else if ((unsigned long long) n * (unsigned long long) s.do_beta < (unsigned long long) s.nodes) {
         return kPhaseExpandContract;
     }
     return phase;
 }
 
-__device__ __forceinline__ int choose_direction(const SearchArgs &s, Cursor c, unsigned n, unsigned long long ve)
-{
-    return direction_rule(s, c.phase, ld64_agent(&s.ctl->fedges[c.step % 3]), n, ve);
-}
 
 // Whether a level is small enough for the single-workgroup fast path.
 // fe: the step's frontier edges (used when they are tracked).
@@ -1732,11 +1732,11 @@ __device__ __forceinline__ void spread_step(const GraphArgs &g, const SearchArgs
 // global queue) until a level's output is empty, too large for the path, or
 // (direction-optimizing) calls for bottom-up.  Leaves the global queue,
 // counters and the fast_* hand-off exactly as the ordinary steps would have.
-// `fv`: visited-edge sum through step c (see choose_direction).  Block 0
-// only; every thread calls it.
+// `fv`: visited-edge sum through step c, fe_in: step c's frontier edges
+// (see direction_rule).  Block 0 only; every thread calls it.
 template <bool PRED, bool EXACT, bool FASTV>
 __device__ __forceinline__ void fast_lds_path(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, int loop,
-                              unsigned long long fv, FastLds<FASTV> &L, bool wait_others)
+                              unsigned long long fv, unsigned long long fe_in, FastLds<FASTV> &L, bool wait_others)
 {
     FastFront &F = L.f;
     const int *in = s.q[c.step & 1];
@@ -1771,6 +1771,8 @@ __device__ __forceinline__ void fast_lds_path(const GraphArgs &g, const SearchAr
     int maxd = -1;
     unsigned n2 = 0;
     unsigned long long fe2 = 0;
+    // frontier edges of the members and of the level before them
+    unsigned long long cur_fe = fe_in, prev_fe = 0;
     const bool ce = loop == kLoopCE, tp = loop == kLoopTP;
     if (ce || tp) {
         // Contract-expand and two-phase enter at a step that contracts a
@@ -1867,9 +1869,11 @@ __device__ __forceinline__ void fast_lds_path(const GraphArgs &g, const SearchAr
         bool go = n2 > 0 && n2 <= (unsigned) kFastCap && n2 <= (unsigned) s.fast_t && fe2 <= kFastEdges;
         if (go && loop == kLoopDO) {
             const unsigned long long fv2 = fv + fe2;
-            if (direction_rule(s, c.phase, fe2, n2, fv2) != c.phase) go = false;
+            if (direction_rule(s, c.phase, fe2, n2, fv2, cur_fe) != c.phase) go = false;
             else fv = fv2;
         }
+        prev_fe = cur_fe;
+        cur_fe = fe2;
         // a hybrid search leaves contract-expand when a step's input
         // exceeds the saturation threshold: the grid takes that step
         if (go && ce && s.sat_t >= 0 && fe2 > (unsigned long long) s.sat_t) go = false;
@@ -1931,6 +1935,7 @@ __device__ __forceinline__ void fast_lds_path(const GraphArgs &g, const SearchAr
         st_agent(&ctl->fast_phase, c.phase);
         // excludes the step the loop top runs next (it adds it)
         __hip_atomic_store(&ctl->fast_vsum, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctl->fast_prev_fe, prev_fe, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -2023,8 +2028,10 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
     int reason = kExitDone;
     // Visited-edge sum up to the current step, kept by every workgroup from
     // counters that no workgroup changes during the step (fedges[step % 3])
-    // -- see choose_direction.  fedges[0] holds the source's degree.
+    // -- see direction_rule.  fedges[0] holds the source's degree.
     unsigned long long vsum = 0;
+    // the previous step's frontier edges (direction rule; grid-uniform too)
+    unsigned long long prev_fe = 0;
 
     // With a folded search start, step 0's loop-top values come from the
     // graph (one source, its degree), not from the control block that block
@@ -2062,7 +2069,8 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
         }
         if (loop == kLoopDO) {
             vsum += fe;
-            int dir = direction_rule(s, c.phase, fe, n, vsum);
+            int dir = direction_rule(s, c.phase, fe, n, vsum, prev_fe);
+            prev_fe = fe;
             if (start && dir != c.phase) {
                 grid_barrier(s.ctl);
                 start = false;
@@ -2131,7 +2139,7 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
                 // frontier kept in LDS between levels
                 if (blockIdx.x == 0) {
                     unsigned long long t0 = s.instrument ? wall_clock64() : 0;
-                    fast_lds_path<PRED, EXACT, FASTV>(g, s, c, n, loop, vsum, shared.fast, lds_path && !first);
+                    fast_lds_path<PRED, EXACT, FASTV>(g, s, c, n, loop, vsum, fe, shared.fast, lds_path && !first);
                     __syncthreads();
                     init_lds(lds);
                     if (s.instrument) busy += wall_clock64() - t0;
@@ -2155,11 +2163,7 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
                     if (loop == kLoopCE && s.sat_t >= 0 && n > (unsigned) s.sat_t) break;
                     // (a small two-phase contraction is the LDS path's entry)
                     if (loop == kLoopTP && c.phase == kPhaseContract && n <= (unsigned) kFastCap) break;
-                    if (loop == kLoopDO) {
-                        unsigned long long fv2 = fv + ld64_agent(&s.ctl->fedges[c.step % 3]);
-                        if (choose_direction(s, c, n, fv2) != c.phase) break;
-                        fv = fv2;
-                    }
+                    // (direction-optimizing searches always take the LDS path)
                 }
                 if (threadIdx.x == 0) {
                     st_agent(&s.ctl->fast_step, c.step);
@@ -2175,6 +2179,7 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
             c.iteration = ld_agent(&s.ctl->fast_iteration);
             c.phase = ld_agent(&s.ctl->fast_phase);
             vsum = ld64_agent(&s.ctl->fast_vsum);
+            prev_fe = ld64_agent(&s.ctl->fast_prev_fe);
             continue;
         }
 
