@@ -285,6 +285,7 @@ struct Lds {
     int ep_lab[kWaves];
     int ep_scan[kWaves];
     unsigned ep_base;
+    int solo;                            // 1: this workgroup runs the step alone (no static tiles)
 };
 
 // ---------------------------------------------------------------------------

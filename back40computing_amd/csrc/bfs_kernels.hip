@@ -32,7 +32,10 @@ template <bool PRED>
 __device__ __forceinline__ void init_lds(Lds<PRED> &lds)
 {
     for (int i = threadIdx.x; i < kHistory; i += kBlock) lds.hist[i] = -1;
-    if (threadIdx.x == 0) lds.cta_owner = -1;
+    if (threadIdx.x == 0) {
+        lds.cta_owner = -1;
+        lds.solo = 0;
+    }
     __syncthreads();
 }
 
@@ -50,22 +53,39 @@ __device__ __forceinline__ unsigned next_item(unsigned *counter, Lds<PRED> &lds,
 // `grab` consecutive tiles, about eight grabs per workgroup when a phase has
 // many tiles, so the single counter address (whose device-scope atomics
 // serialise) stays off the critical path of large levels.
+// A phase of at most one tile per workgroup that every workgroup runs needs
+// no counter: workgroup b takes tile b (grab 0), and the two round trips of
+// the counter atomics (the grab, and the one that finds nothing left) leave
+// the level's critical path.  A workgroup running a step alone (lds.solo)
+// grabs every tile itself.
 struct TileGrab {
-    unsigned next, end, grab;
+    unsigned next, end, grab, tiles;
 };
 
-__device__ __forceinline__ TileGrab tile_grab(const SearchArgs &s, unsigned items)
+// tiles per counter atomic of a dynamic assignment
+__device__ __forceinline__ unsigned tile_grab_size(const SearchArgs &s, unsigned tiles)
 {
-    unsigned tiles = (items + kBlock - 1) / kBlock;
-    unsigned g = tiles / (gridDim.x * 8u);
-    g = max(1u, min((unsigned) s.grab_max, g));
-    return TileGrab{0u, 0u, g};
+    return max(1u, min((unsigned) s.grab_max, tiles / (gridDim.x * 8u)));
 }
 
-// Next tile index of the workgroup (workgroup-uniform).
+template <bool PRED>
+__device__ __forceinline__ TileGrab tile_grab(const SearchArgs &s, unsigned items, const Lds<PRED> &lds)
+{
+    unsigned tiles = (items + kBlock - 1) / kBlock;
+    if (tiles <= gridDim.x && !lds.solo)
+        return TileGrab{blockIdx.x, blockIdx.x < tiles ? blockIdx.x + 1 : blockIdx.x, 0u, tiles};
+    return TileGrab{0u, 0u, tile_grab_size(s, tiles), tiles};
+}
+
+// Next tile index of the workgroup (workgroup-uniform); with a static
+// assignment, the phase's tile count once the workgroup's tile is done.
 template <bool PRED>
 __device__ __forceinline__ unsigned next_tile(unsigned *counter, Lds<PRED> &lds, TileGrab &tg)
 {
+    if (tg.grab == 0) {
+        __syncthreads();  // the LDS reuse fence of a grab
+        return tg.next == tg.end ? tg.tiles : tg.next++;
+    }
     if (tg.next == tg.end) {
         tg.next = next_item(counter, lds, tg.grab);
         tg.end = tg.next + tg.grab;
@@ -213,7 +233,7 @@ template <bool PRED, typename RowOf, typename Edge>
 __device__ __forceinline__ void tile_phase(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n,
                                            const HubList &hub, Lds<PRED> &lds, RowOf &&row_of, Edge &&edge)
 {
-    TileGrab tg = tile_grab(s, n);
+    TileGrab tg = tile_grab(s, n, lds);
     while (true) {
         unsigned base = next_tile(&s.ctl->steal[c.step & 1], lds, tg) * kBlock;
         if (base >= n) break;
@@ -242,7 +262,7 @@ __device__ __forceinline__ void hub_phase_chunks(const GraphArgs &g, const Searc
     unsigned total = lastent.first_chunk + (unsigned) ((lastent.end - lastent.beg + chunk - 1) / chunk);
     // guided grabs of consecutive chunks, as for tiles: a level of many hub
     // chunks must not queue on the one chunk counter
-    TileGrab tg{0u, 0u, max(1u, min((unsigned) s.grab_max, total / (gridDim.x * 8u)))};
+    TileGrab tg{0u, 0u, max(1u, min((unsigned) s.grab_max, total / (gridDim.x * 8u))), 0u};
     unsigned hint = 0;  // thread 0: entry of this workgroup's previous chunk
     while (true) {
         unsigned ch = next_tile(&s.ctl->hub_steal[c.step & 1], lds, tg);
@@ -701,7 +721,8 @@ __device__ __forceinline__ void chained_step(const SearchArgs &s, Cursor c, unsi
     int *tile_off = &lds.sc_excl[0][0];
     const int lane = lane_id(), wave = threadIdx.x / kWave;
     const unsigned tiles = (n + kBlock - 1) / kBlock;
-    const unsigned g = min(tile_grab(s, n).grab, kChainGrabMax);
+    // (grabs in increasing order, as the look-back needs: always dynamic)
+    const unsigned g = min(tile_grab_size(s, tiles), kChainGrabMax);
     const unsigned grabs = (tiles + g - 1) / g;
     const unsigned epoch = (s.scan_epoch + (unsigned) c.step) & 0x3FFFFFFFu;
     while (true) {
@@ -776,7 +797,7 @@ __device__ __forceinline__ void chained_step(const SearchArgs &s, Cursor c, unsi
 }
 
 template <bool PRED, bool EXACT>
-__device__ void body_contract(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds)
+__device__ __forceinline__ void body_contract(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds)
 {
     (void) g;
     WaveQueue<PRED> wq = make_queue<PRED>(s, c.step, lds);
@@ -786,7 +807,7 @@ __device__ void body_contract(const GraphArgs &g, const SearchArgs &s, Cursor c,
     if (s.tile_status) {
         chained_step<PRED, EXACT, false>(s, c, n, lds, labeled);
     } else {
-        TileGrab tg = tile_grab(s, n);
+        TileGrab tg = tile_grab(s, n, lds);
         while (true) {
             unsigned base = next_tile(&s.ctl->steal[c.step & 1], lds, tg) * kBlock;
             if (base >= n) break;
@@ -809,7 +830,7 @@ __device__ void body_contract(const GraphArgs &g, const SearchArgs &s, Cursor c,
 }
 
 template <bool PRED, bool EXACT>
-__device__ void body_filter(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds)
+__device__ __forceinline__ void body_filter(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, Lds<PRED> &lds)
 {
     (void) g;
     WaveQueue<PRED> wq = make_queue<PRED>(s, c.step, lds);
@@ -819,7 +840,7 @@ __device__ void body_filter(const GraphArgs &g, const SearchArgs &s, Cursor c, u
         bool unused = false;
         chained_step<PRED, EXACT, true>(s, c, n, lds, unused);
     } else {
-        TileGrab tg = tile_grab(s, n);
+        TileGrab tg = tile_grab(s, n, lds);
         while (true) {
             unsigned base = next_tile(&s.ctl->steal[c.step & 1], lds, tg) * kBlock;
             if (base >= n) break;
@@ -952,7 +973,7 @@ __device__ __forceinline__ void body_expand_mark(const GraphArgs &g, const Searc
         int *tpar = lds.wq[1];
         int *toff = lds.wq[2];
         static_assert(kWaveQueueCap >= kBlock && kWaves >= 3, "tile tables fit the staging area");
-        TileGrab tg = tile_grab(s, n);
+        TileGrab tg = tile_grab(s, n, lds);
         while (true) {
             const unsigned base = next_tile(&s.ctl->steal[c.step & 1], lds, tg) * kBlock;
             if (base >= n) break;
@@ -2147,6 +2168,9 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
             } else if (blockIdx.x == 0) {
                 unsigned long long t0 = s.instrument ? wall_clock64() : 0;
                 unsigned long long fv = vsum;  // sum through the step being run
+                // (the step bodies grab every tile: see tile_grab)
+                if (threadIdx.x == 0) lds.solo = 1;
+                __syncthreads();
                 while (true) {
                     if (threadIdx.x == 0) prepare_step(s.ctl, c, n);
                     loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 0, false);
@@ -2171,6 +2195,7 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
                     st_agent(&s.ctl->fast_phase, c.phase);
                     // excludes the step the loop top runs next (it adds it)
                     __hip_atomic_store(&s.ctl->fast_vsum, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    lds.solo = 0;  // (the grid barrier below orders it)
                 }
                 if (s.instrument) busy += wall_clock64() - t0;
             }
