@@ -18,6 +18,7 @@ namespace b40c_hip {
 constexpr int kWave = 64;
 constexpr int kHistory = 1024;                 // LDS history-hash entries per workgroup
 constexpr int kBarrierFanIn = 16;              // first-level barrier counters
+constexpr int kMaxXcdGrid = 1024;              // largest grid whose barriers may use XCD leaves
 constexpr int kHubChunk = 2048;                // edges per grid-tier chunk
 constexpr int kLevelLog = 64;                  // steps with recorded length/clock
 
@@ -63,6 +64,7 @@ struct alignas(256) Control {
     unsigned bar_release[kBarrierFanIn * 32];  // one 128-byte line per leaf release word
     unsigned level_len[kLevelLog];             // input length of each step
     unsigned long long level_clock[kLevelLog]; // wall clock at the start of each step
+    unsigned xcd_of[kMaxXcdGrid];              // XCD of each workgroup of a persistent launch
 };
 
 // Bytes of the control block's status head (counters, flags, statistics):
@@ -526,7 +528,39 @@ __device__ __forceinline__ void gather_rows(int beg, int end, int par, const int
 // every workgroup's pre-barrier writes happen-before every post-barrier read.
 // Replaces util::GlobalBarrier (b40c/util/global_barrier.cuh:80-126).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void grid_barrier(Control *ctl)
+// Which leaf a workgroup arrives on, how many workgroups share it, and how
+// many leaves there are.  By block (b % kBarrierFanIn), every workgroup
+// writes back its XCD's L2 before arriving.  With XCD leaves (shared), the
+// leaf is the workgroup's XCD and only the leaf's last arriver writes back
+// that XCD's L2 -- once for all of them: the L1 is write-through, so a
+// workgroup's stores are in its XCD's L2 once they have completed, which
+// every workgroup waits for before it arrives.
+struct BarrierShape {
+    unsigned leaf, leaf_size, leaves, shared;
+};
+static_assert(kBarrierFanIn >= 8, "an XCD leaf indexes the leaf words");
+
+__device__ __forceinline__ BarrierShape barrier_shape_by_block()
+{
+    const unsigned nblocks = gridDim.x;
+    BarrierShape b;
+    b.leaf = blockIdx.x % kBarrierFanIn;
+    b.leaves = nblocks < (unsigned) kBarrierFanIn ? nblocks : (unsigned) kBarrierFanIn;
+    // workgroups on this leaf: ceil((nblocks - leaf) / fan_in)
+    b.leaf_size = (nblocks - b.leaf + kBarrierFanIn - 1) / kBarrierFanIn;
+    b.shared = 0u;
+    return b;
+}
+
+// The XCD (accelerator complex die) the calling wave runs on.
+__device__ __forceinline__ unsigned xcc_id()
+{
+    unsigned x;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
+    return x & 7u;
+}
+
+__device__ __forceinline__ void grid_barrier(Control *ctl, const BarrierShape &shape)
 {
     // Every wave drains its own outstanding memory operations first:
     // __syncthreads() alone does not wait for other waves' stores, and the
@@ -536,11 +570,7 @@ __device__ __forceinline__ void grid_barrier(Control *ctl)
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
-        const unsigned nblocks = gridDim.x;
-        const unsigned leaf = blockIdx.x % kBarrierFanIn;
-        const unsigned leaves = nblocks < (unsigned) kBarrierFanIn ? nblocks : (unsigned) kBarrierFanIn;
-        // Workgroups on this leaf: ceil((nblocks - leaf) / fan_in).
-        const unsigned leaf_size = (nblocks - leaf + kBarrierFanIn - 1) / kBarrierFanIn;
+        const unsigned leaf = shape.leaf, leaves = shape.leaves, leaf_size = shape.leaf_size;
         unsigned *release = &ctl->bar_release[leaf * 32];
         // The generation comes from the one global word, which only the last
         // arriver of a barrier advances (after every workgroup has read it
@@ -548,7 +578,10 @@ __device__ __forceinline__ void grid_barrier(Control *ctl)
         // Leaf words of leaves a smaller earlier grid never used may lag:
         // they are compared with gen + 1, never with each other.
         const unsigned gen = __hip_atomic_load(&ctl->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        // (with XCD leaves the leaf's last arriver writes the L2 back; the
+        // generation must still be read before this arrival)
+        if (shape.shared) __builtin_amdgcn_s_waitcnt(0);
+        else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         unsigned *lc = &ctl->bar_leaf[leaf * 32];
         unsigned arrived = __hip_atomic_fetch_add(lc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         bool last = false;
@@ -585,20 +618,66 @@ __device__ __forceinline__ void grid_barrier(Control *ctl)
     __syncthreads();
 }
 
-// Thread 0 of block 0 only: waits until every other workgroup has arrived at
-// the grid barrier in progress (block 0 itself has not), i.e. until every
-// other workgroup is done with whatever it did before that barrier.  Leaf 0
-// then holds all its workgroups but block 0 and the root all leaves but
-// leaf 0; neither count can move on before block 0 arrives, so the wait
-// ends exactly once they are reached.
-__device__ __forceinline__ void wait_other_arrivals(Control *ctl)
+__device__ __forceinline__ void grid_barrier(Control *ctl) { grid_barrier(ctl, barrier_shape_by_block()); }
+
+// Thread 0 of block 0 only (shape: block 0's): waits until every other
+// workgroup has arrived at the grid barrier in progress (block 0 itself has
+// not), i.e. until every other workgroup is done with whatever it did
+// before that barrier.  Block 0's leaf then holds all its workgroups but
+// block 0 and the root all leaves but that one; neither count can move on
+// before block 0 arrives, so the wait ends exactly once they are reached.
+__device__ __forceinline__ void wait_other_arrivals(Control *ctl, const BarrierShape &shape)
 {
-    const unsigned nblocks = gridDim.x;
-    const unsigned leaves = nblocks < (unsigned) kBarrierFanIn ? nblocks : (unsigned) kBarrierFanIn;
-    const unsigned leaf0 = (nblocks + kBarrierFanIn - 1) / kBarrierFanIn;  // workgroups on leaf 0
-    while (__hip_atomic_load(&ctl->bar_root, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != leaves - 1 ||
-           __hip_atomic_load(&ctl->bar_leaf[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != leaf0 - 1)
+    while (__hip_atomic_load(&ctl->bar_root, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != shape.leaves - 1 ||
+           __hip_atomic_load(&ctl->bar_leaf[shape.leaf * 32], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
+               shape.leaf_size - 1)
         __builtin_amdgcn_s_sleep(2);
+}
+
+// Thread 0 of every workgroup of a persistent launch (gridDim.x <=
+// kMaxXcdGrid), before its first barrier: records the workgroup's XCD.
+__device__ __forceinline__ void xcd_register(Control *ctl) { ctl->xcd_of[blockIdx.x] = xcc_id(); }
+
+// Thread 0, after the launch's first barrier (every workgroup recorded its
+// XCD before arriving at it): XCD leaves when all eight XCDs hold
+// workgroups of this launch, else leaves by block.  The counts are kept
+// in 16-bit fields, four XCDs to a word.
+__device__ __forceinline__ BarrierShape xcd_shape(Control *ctl)
+{
+    const unsigned x = xcc_id();
+    unsigned long long lo = 0, hi = 0;
+    const uint4 *rec = reinterpret_cast<const uint4 *>(ctl->xcd_of);
+    const unsigned words = gridDim.x / 4, tail = gridDim.x % 4;
+    for (unsigned i = 0; i < words; i++) {
+        const uint4 q = rec[i];
+        const unsigned v[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const unsigned y = v[k] & 7u;
+            if (y < 4) lo += 1ull << (16 * y);
+            else hi += 1ull << (16 * (y - 4));
+        }
+    }
+    for (unsigned i = 4 * words; i < 4 * words + tail; i++) {
+        const unsigned y = ctl->xcd_of[i] & 7u;
+        if (y < 4) lo += 1ull << (16 * y);
+        else hi += 1ull << (16 * (y - 4));
+    }
+    bool all = true;
+    unsigned mine = 0;
+#pragma unroll
+    for (unsigned k = 0; k < 8; k++) {
+        const unsigned c = (unsigned) (((k < 4 ? lo : hi) >> (16 * (k & 3))) & 0xFFFFull);
+        all = all && c > 0;
+        if (k == x) mine = c;
+    }
+    if (!all) return barrier_shape_by_block();
+    BarrierShape b;
+    b.leaf = x;
+    b.leaf_size = mine;
+    b.leaves = 8u;
+    b.shared = 1u;
+    return b;
 }
 
 }  // inline namespace (workgroup size)

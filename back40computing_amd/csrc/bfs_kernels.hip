@@ -1757,7 +1757,8 @@ __device__ __forceinline__ void spread_step(const GraphArgs &g, const SearchArgs
 // (see direction_rule).  Block 0 only; every thread calls it.
 template <bool PRED, bool EXACT, bool FASTV>
 __device__ __forceinline__ void fast_lds_path(const GraphArgs &g, const SearchArgs &s, Cursor c, unsigned n, int loop,
-                              unsigned long long fv, unsigned long long fe_in, FastLds<FASTV> &L, bool wait_others)
+                              unsigned long long fv, unsigned long long fe_in, FastLds<FASTV> &L, bool wait_others,
+                              const BarrierShape &bshape)
 {
     FastFront &F = L.f;
     const int *in = s.q[c.step & 1];
@@ -1925,7 +1926,7 @@ __device__ __forceinline__ void fast_lds_path(const GraphArgs &g, const SearchAr
     // be reading at the top of the step the path started from, unless they
     // have all arrived at the barrier that ends the path.
     if (wait_others) {
-        if (threadIdx.x == 0) wait_other_arrivals(s.ctl);
+        if (threadIdx.x == 0) wait_other_arrivals(s.ctl, bshape);
         __syncthreads();
     }
     // The next step (c) reads its input from the global queue: copy the
@@ -2038,6 +2039,12 @@ __device__ __forceinline__ void publish_status(const SearchArgs &s)
 // ---------------------------------------------------------------------------
 // One instantiation per strategy (LOOP), so profiles attribute time to the
 // strategy and the level loop carries no dead strategy branches.
+// Most work of a step whose grid barrier leaves the L2 write-back to each
+// XCD's last arriver, and the barrier of a launch from which XCD leaves
+// are used.
+constexpr unsigned long long kLightWork = 1ull << 19;
+constexpr unsigned kXcdBarrierFrom = 32;
+
 template <bool PRED, bool EXACT, int LOOP, bool FASTV>
 __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchArgs s, Cursor c)
 {
@@ -2059,6 +2066,28 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
     // 0 is still writing, and the step's first grid barrier orders the
     // label rewrite before any claim.
     bool start = s.init_src >= 0;
+    // Grid barriers: leaves by block, and from the kXcdBarrierFrom-th
+    // barrier of a long launch on (a high-diameter search) the XCDs, counted
+    // from the records every workgroup left before its first barrier
+    // (thread 0's shape is the one used; the count costs a few
+    // microseconds, which a short launch does not win back).  After a light
+    // phase (little written) only each XCD's last arriver then writes its L2
+    // back; after a heavy one every workgroup still writes back its XCD's L2
+    // before arriving, which spreads the write-back over the arrivals
+    // instead of leaving it all to the last one.
+    BarrierShape bshape = barrier_shape_by_block();
+    bool shape_pending = gridDim.x >= 8u && gridDim.x <= (unsigned) kMaxXcdGrid;
+    unsigned barriers = 0;
+    if (shape_pending && threadIdx.x == 0) xcd_register(s.ctl);
+    auto bar = [&](bool light) {
+        BarrierShape b = bshape;
+        b.shared = bshape.shared && light ? 1u : 0u;
+        grid_barrier(s.ctl, b);
+        if (shape_pending && ++barriers == kXcdBarrierFrom) {
+            shape_pending = false;
+            if (threadIdx.x == 0) bshape = xcd_shape(s.ctl);
+        }
+    };
     if (start) search_prologue(g, s, LOOP == kLoopDO);
     // Frontier bitmaps of a direction-optimizing loop (grid-uniform state):
     // which are not known to be clear, and the step whose frontier one of
@@ -2093,17 +2122,17 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
             int dir = direction_rule(s, c.phase, fe, n, vsum, prev_fe);
             prev_fe = fe;
             if (start && dir != c.phase) {
-                grid_barrier(s.ctl);
+                bar(false);
                 start = false;
             }
             if (dir == kPhaseBottomUp && c.phase != kPhaseBottomUp) {
                 if (fbm_exact_step != c.step) {
                     if (fbm_dirty & (1u << (c.iteration & 1))) {
                         frontier_bitmap_pass(g, s, c, n, 0);
-                        grid_barrier(s.ctl);
+                        bar(false);
                     }
                     frontier_bitmap_pass(g, s, c, n, 1);
-                    grid_barrier(s.ctl);
+                    bar(false);
                 }
                 fbm_dirty = 3u;  // bottom-up levels write both
             } else if (dir != kPhaseBottomUp && c.phase == kPhaseBottomUp) {
@@ -2133,10 +2162,10 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
         const bool from_bitmap = bu_exit && !fast && !mark && !spread;
         if (bu_exit && !from_bitmap) {
             queue_from_bitmap<PRED>(g, s, c, lds);  // clears both bitmaps
-            grid_barrier(s.ctl);
+            bar(false);
             fbm_dirty = 0u;
         }
-        if (start && !fast) grid_barrier(s.ctl);
+        if (start && !fast) bar(false);
         const bool first = start;
         start = false;
         if (fast) {
@@ -2155,12 +2184,13 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
             // after a folded search start, whose label rewrite the entry
             // barrier orders before the first claim.
             const bool lds_path = LOOP == kLoopEC || LOOP == kLoopDO || contract_lds;
-            if (!lds_path || first) grid_barrier(s.ctl);
+            if (!lds_path || first) bar(!first);
             if (lds_path) {
                 // frontier kept in LDS between levels
                 if (blockIdx.x == 0) {
                     unsigned long long t0 = s.instrument ? wall_clock64() : 0;
-                    fast_lds_path<PRED, EXACT, FASTV>(g, s, c, n, loop, vsum, fe, shared.fast, lds_path && !first);
+                    fast_lds_path<PRED, EXACT, FASTV>(g, s, c, n, loop, vsum, fe, shared.fast, lds_path && !first,
+                                                      bshape);
                     __syncthreads();
                     init_lds(lds);
                     if (s.instrument) busy += wall_clock64() - t0;
@@ -2199,7 +2229,7 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
                 }
                 if (s.instrument) busy += wall_clock64() - t0;
             }
-            grid_barrier(s.ctl);
+            bar(true);
             c.step = ld_agent(&s.ctl->fast_step);
             c.iteration = ld_agent(&s.ctl->fast_iteration);
             c.phase = ld_agent(&s.ctl->fast_phase);
@@ -2232,13 +2262,17 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
             loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 0, true, next_bits, from_bitmap);
         }
         if (s.instrument) busy += wall_clock64() - t0;
-        grid_barrier(s.ctl);
+        // (light: a top-down or contract/expand step of at most
+        // kLightWork frontier edges, or items when edges are not tracked)
+        const bool light_step = c.phase != kPhaseBottomUp && !mark &&
+                                (s.track_edges ? fe : (unsigned long long) n) <= kLightWork;
+        bar(light_step);
         if (!spread && gathers(c.phase) && hubs_pending(s, c.step)) {
             t0 = s.instrument ? wall_clock64() : 0;
             if (mark) body_expand_mark<PRED, EXACT>(g, s, c, n, lds, 1, true);
             else loop_body<PRED, EXACT, LOOP>(c.phase, g, s, c, n, lds, 1, true, next_bits, from_bitmap);
             if (s.instrument) busy += wall_clock64() - t0;
-            grid_barrier(s.ctl);
+            bar(false);
         }
         if (mark) {
             t0 = s.instrument ? wall_clock64() : 0;
@@ -2250,7 +2284,7 @@ __global__ __launch_bounds__(kBlock) void persistent_kernel(GraphArgs g, SearchA
             }
             mark_compact<PRED, EXACT>(g, s, c, lds, fbm_out);
             if (s.instrument) busy += wall_clock64() - t0;
-            grid_barrier(s.ctl);
+            bar(false);
         }
         // (fbm_dirty kept this step's claims out of the bitmap it cleared)
         if (from_bitmap) fbm_dirty = 0u;
